@@ -1,0 +1,1221 @@
+/*
+ * ORACLE -- test infrastructure only; see eph_oracle.h.
+ *
+ * Plain-C restatement of the reference CPU path.  Build with
+ * -ffp-contract=off and no fast-math: every operator below is one IEEE
+ * operation in the order the reference C++ expression evaluates it.
+ * Vector division multiplies by the reciprocal (src/math/vec_t.h:197-203,
+ * 262-265), sums of three products fold left ((a+b)+c), and double-double
+ * follows src/math/dfloat_t.h.
+ */
+#include "eph_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define INL static inline
+
+/* src/math/basic_math.h:7-31 */
+static const double C_LIGHT = 299792458.0;
+#define C2_LIGHT (C_LIGHT * C_LIGHT)
+static const double PI_REF = 3.1415926535897932;
+static const double G_NEWTON = 6.67430E-11;
+static const double DEGREE = 3.1415926535897932 / 180;
+
+/* ------------------------------------------------------------------------- */
+/* double-double, src/math/dfloat_t.h:83-175                                  */
+/* ------------------------------------------------------------------------- */
+INL o_dd dd(double h) { o_dd r = {h, 0.0}; return r; }
+INL double dd2d(o_dd a) { return a.hi + a.lo; }
+
+INL o_dd dd_add(o_dd x, o_dd y) {           /* dfloat_t.h:83-89 */
+    o_dd re;
+    double r = x.hi + y.hi, t = r - x.hi;
+    double e = (x.hi - (r - t)) + (y.hi - t);
+    e = e + (x.lo + y.lo);
+    re.hi = r + e;
+    re.lo = e - (re.hi - r);
+    return re;
+}
+INL o_dd dd_sub(o_dd x, o_dd y) {           /* dfloat_t.h:91-97 */
+    o_dd re;
+    double r = x.hi - y.hi, t = r - x.hi;
+    double e = (x.hi - (r - t)) - (y.hi + t);
+    e = e + (x.lo - y.lo);
+    re.hi = r + e;
+    re.lo = e - (re.hi - r);
+    return re;
+}
+INL o_dd dd_mul(o_dd x, o_dd y) {           /* dfloat_t.h:99-105 */
+    o_dd re;
+    double r = x.hi * y.hi;
+    double e = fma(x.hi, y.hi, -r);
+    e = e + (x.hi * y.lo + y.hi * x.lo);
+    re.hi = r + e;
+    re.lo = e - (re.hi - r);
+    return re;
+}
+INL o_dd dd_div(o_dd x, o_dd y) {           /* dfloat_t.h:107-114 */
+    o_dd re;
+    double h = x.hi / y.hi;
+    double sh = h * y.hi;
+    double sl = fma(h, y.hi, -sh);
+    double l = (x.hi - sh - sl + x.lo - h * y.lo) / y.hi;
+    re.hi = h + l;
+    re.lo = l - (re.hi - h);
+    return re;
+}
+INL o_dd dd_sqrt(o_dd x) {                  /* dfloat_t.h:129-134 */
+    o_dd re = dd(sqrt(x.hi));
+    if (re.hi > 0)
+        re = dd_sub(re, dd_div(dd_sub(re, dd_div(x, re)), dd(2.0)));
+    return re;
+}
+INL int dd_is_finite(o_dd a) { return !((0 * a.hi) != 0 || (0 * a.lo) != 0); }
+
+/* ------------------------------------------------------------------------- */
+/* 3-vectors and 3x3 matrices of double, src/math/vec_t.h                     */
+/* ------------------------------------------------------------------------- */
+INL o_v3 v3(double x, double y, double z) { o_v3 r = {x, y, z}; return r; }
+INL o_v3 vadd(o_v3 a, o_v3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+INL o_v3 vsub(o_v3 a, o_v3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+INL o_v3 vneg(o_v3 a) { return v3(-a.x, -a.y, -a.z); }
+INL o_v3 vscale(double s, o_v3 a) { return v3(s * a.x, s * a.y, s * a.z); }
+INL o_v3 vdivs(o_v3 a, double s) { double rs = 1 / s; return v3(a.x * rs, a.y * rs, a.z * rs); }
+INL o_v3 vcross(o_v3 a, o_v3 b) {
+    return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+INL double vdot(o_v3 a, o_v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+/* hypot_t<double>::_normalize, src/math/vec_t.h:53-81 */
+static const double HYP_HIGH = 0x1p960, HYP_MUL = 0x1p768;
+#define HYP_DIV (1 / HYP_MUL)
+#define HYP_LOW (1 / HYP_HIGH)
+static double normalize3(double *px, double *py, double *pz) {
+    double x = *px, y = *py, z = *pz;
+    double r = x * x + y * y + z * z;
+    if (r < HYP_LOW) {
+        x *= HYP_MUL; y *= HYP_MUL; z *= HYP_MUL;
+        r = x * x + y * y + z * z;
+        if (r == 0) {
+            *px = 0; *py = 0; *pz = 1;
+            return r;
+        }
+        double s = sqrt(r), rs = 1 / s;
+        *px = x * rs; *py = y * rs; *pz = z * rs;
+        return HYP_DIV * s;
+    }
+    if (!(r < HYP_HIGH)) {
+        x *= HYP_DIV; y *= HYP_DIV; z *= HYP_DIV;
+        r = x * x + y * y + z * z;
+        double s = sqrt(r), rs = 1 / s;
+        *px = x * rs; *py = y * rs; *pz = z * rs;
+        return HYP_MUL * s;
+    }
+    double s = sqrt(r), rs = 1 / s;
+    *px = x * rs; *py = y * rs; *pz = z * rs;
+    return s;
+}
+static double normalize2(double *pa, double *pb) {
+    double a = *pa, b = *pb;
+    double r = a * a + b * b;
+    if (r < HYP_LOW) {
+        a *= HYP_MUL; b *= HYP_MUL;
+        r = a * a + b * b;
+        if (r == 0) {
+            *pa = 0; *pb = 1;
+            return r;
+        }
+        double s = sqrt(r), rs = 1 / s;
+        *pa = a * rs; *pb = b * rs;
+        return HYP_DIV * s;
+    }
+    if (!(r < HYP_HIGH)) {
+        a *= HYP_DIV; b *= HYP_DIV;
+        r = a * a + b * b;
+        double s = sqrt(r), rs = 1 / s;
+        *pa = a * rs; *pb = b * rs;
+        return HYP_MUL * s;
+    }
+    double s = sqrt(r), rs = 1 / s;
+    *pa = a * rs; *pb = b * rs;
+    return s;
+}
+INL double vnormalize(o_v3 *a) { return normalize3(&a->x, &a->y, &a->z); }
+INL o_v3 vunit(o_v3 a) { vnormalize(&a); return a; }
+INL o_v3 vasc_node(o_v3 a) {               /* vec_t.h:226-230 */
+    o_v3 p = v3(-a.y, a.x, 0);
+    normalize2(&p.y, &p.x);
+    return p;
+}
+
+INL o_m3 m3c(o_v3 x, o_v3 y, o_v3 z) { o_m3 m = {x, y, z}; return m; }
+INL o_m3 mzero(void) { return m3c(v3(0, 0, 0), v3(0, 0, 0), v3(0, 0, 0)); }
+INL o_v3 mmulv(o_m3 a, o_v3 b) {            /* mat%vec, vec_t.h:476-478 */
+    return vadd(vadd(vscale(b.x, a.x), vscale(b.y, a.y)), vscale(b.z, a.z));
+}
+INL o_v3 vmulm(o_v3 a, o_m3 b) { return v3(vdot(a, b.x), vdot(a, b.y), vdot(a, b.z)); }
+INL o_m3 mmulm(o_m3 a, o_m3 b) { return m3c(mmulv(a, b.x), mmulv(a, b.y), mmulv(a, b.z)); }
+INL o_m3 mouter(o_v3 a, o_v3 b) { return m3c(vscale(b.x, a), vscale(b.y, a), vscale(b.z, a)); }
+INL o_m3 mscale(double s, o_m3 a) { return m3c(vscale(s, a.x), vscale(s, a.y), vscale(s, a.z)); }
+INL o_m3 madd(o_m3 a, o_m3 b) { return m3c(vadd(a.x, b.x), vadd(a.y, b.y), vadd(a.z, b.z)); }
+INL o_m3 msub(o_m3 a, o_m3 b) { return m3c(vsub(a.x, b.x), vsub(a.y, b.y), vsub(a.z, b.z)); }
+INL o_m3 mneg(o_m3 a) { return m3c(vneg(a.x), vneg(a.y), vneg(a.z)); }
+INL o_m3 madddiag(o_m3 a, double s) { a.x.x += s; a.y.y += s; a.z.z += s; return a; }
+INL o_m3 msubdiag(o_m3 a, double s) { a.x.x -= s; a.y.y -= s; a.z.z -= s; return a; }
+INL o_m3 mtranspose(o_m3 a) {
+    return m3c(v3(a.x.x, a.y.x, a.z.x), v3(a.x.y, a.y.y, a.z.y), v3(a.x.z, a.y.z, a.z.z));
+}
+INL double mdet(o_m3 a) { return vdot(vcross(a.x, a.y), a.z); }
+INL o_m3 madjoint(o_m3 m) {                 /* vec_t.h:443-449 */
+    return m3c(v3(m.y.y * m.z.z - m.y.z * m.z.y, m.z.y * m.x.z - m.x.y * m.z.z, m.x.y * m.y.z - m.x.z * m.y.y),
+               v3(m.y.z * m.z.x - m.y.x * m.z.z, m.z.z * m.x.x - m.x.z * m.z.x, m.x.z * m.y.x - m.x.x * m.y.z),
+               v3(m.y.x * m.z.y - m.y.y * m.z.x, m.z.x * m.x.y - m.x.x * m.z.y, m.x.x * m.y.y - m.x.y * m.y.x));
+}
+INL o_m3 minverse(o_m3 m) {                 /* vec_t.h:450-454 */
+    o_m3 a = madjoint(m);
+    double rdet = 1 / (a.x.z * m.z.x + a.y.z * m.z.y + a.z.z * m.z.z);
+    return m3c(vscale(rdet, a.x), vscale(rdet, a.y), vscale(rdet, a.z));
+}
+INL o_m3 mtoworld(o_m3 s, o_m3 a) {         /* vec_t.h:335-338 */
+    o_m3 sa = mmulm(s, a);
+    return m3c(vadd(vadd(vscale(s.x.x, sa.x), vscale(s.y.x, sa.y)), vscale(s.z.x, sa.z)),
+               vadd(vadd(vscale(s.x.y, sa.x), vscale(s.y.y, sa.y)), vscale(s.z.y, sa.z)),
+               vadd(vadd(vscale(s.x.z, sa.x), vscale(s.y.z, sa.y)), vscale(s.z.z, sa.z)));
+}
+INL o_m3 mtolocal(o_m3 s, o_m3 a) {         /* vec_t.h:330-333 */
+    o_m3 sta = m3c(vmulm(a.x, s), vmulm(a.y, s), vmulm(a.z, s));
+    return mmulm(sta, s);
+}
+static o_m3 rotation_matrix(o_v3 w, double t) { /* vec_t.h:540-559 */
+    double x = w.x, y = w.y, z = w.z;
+    double wt = normalize3(&x, &y, &z) * t;
+    double st = sin(wt), st222 = sin(wt / 2);
+    st222 *= 2 * st222;
+    o_m3 dr = m3c(v3(x * x - 1, x * y, x * z), v3(x * y, y * y - 1, y * z), v3(x * z, y * z, z * z - 1));
+    dr = mscale(st222, dr);
+    dr.x.y += z * st;
+    dr.y.x -= z * st;
+    dr.x.z -= y * st;
+    dr.z.x += y * st;
+    dr.y.z += x * st;
+    dr.z.y -= x * st;
+    return dr;
+}
+INL o_m3 from_harmonics(double j2, double c21, double c22, double s21, double s22) {
+    o_m3 m;                                 /* vec_t.h:522-530 */
+    m.x.x = 3 * c22 + j2 / 2;
+    m.y.x = m.x.y = 3 * s22;
+    m.z.x = m.x.z = 3 * c21 / 2;
+    m.y.y = -3 * c22 + j2 / 2;
+    m.z.y = m.y.z = 3 * s21 / 2;
+    m.z.z = -j2;
+    return m;
+}
+
+/* double-double vectors and matrices */
+INL o_ddv3 ddv(o_v3 a) { o_ddv3 r = {dd(a.x), dd(a.y), dd(a.z)}; return r; }
+INL o_v3 ddv2v(o_ddv3 a) { return v3(dd2d(a.x), dd2d(a.y), dd2d(a.z)); }
+INL o_ddv3 ddvadd(o_ddv3 a, o_ddv3 b) { o_ddv3 r = {dd_add(a.x, b.x), dd_add(a.y, b.y), dd_add(a.z, b.z)}; return r; }
+INL o_v3 ddvdiff(o_ddv3 a, o_ddv3 b) {      /* fast_mpvec(a-b) */
+    return v3(dd2d(dd_sub(a.x, b.x)), dd2d(dd_sub(a.y, b.y)), dd2d(dd_sub(a.z, b.z)));
+}
+INL o_ddv3 ddvscale(o_dd s, o_ddv3 a) { o_ddv3 r = {dd_mul(s, a.x), dd_mul(s, a.y), dd_mul(s, a.z)}; return r; }
+INL int ddv_is_finite(o_ddv3 a) { return dd_is_finite(a.z) && dd_is_finite(a.y) && dd_is_finite(a.x); }
+INL o_m3 ddm2m(o_ddm3 a) { return m3c(ddv2v(a.x), ddv2v(a.y), ddv2v(a.z)); }
+INL o_ddm3 ddmadd(o_ddm3 a, o_m3 b) {
+    o_ddm3 r = {ddvadd(a.x, ddv(b.x)), ddvadd(a.y, ddv(b.y)), ddvadd(a.z, ddv(b.z))};
+    return r;
+}
+INL o_ddm3 ddm(o_m3 a) { o_ddm3 r = {ddv(a.x), ddv(a.y), ddv(a.z)}; return r; }
+
+INL double smax(double a, double b) { return (a < b) ? b : a; }        /* std::max */
+INL void checked_maximize(double *a, double b) {                       /* basic_math.h:34-46 */
+    double cur = *a;
+    *a = (cur != cur || b < cur) ? cur : b;
+}
+
+/* ------------------------------------------------------------------------- */
+/* geopotential::sum, src/physics/geopotential.impl:56-106                     */
+/* ------------------------------------------------------------------------- */
+#define GP_J(n) ((n) * (n) - 4)
+void o_gp_sum(const o_gp *gp, double R, const double *pr, double *out) {
+    o_v3 r = v3(pr[0], pr[1], pr[2]);
+    double rho2 = r.x * r.x + r.y * r.y;
+    double rr2 = 1 / (rho2 + r.z * r.z);
+    double rho1 = sqrt(rho2);
+    double rr1 = sqrt(rr2);
+    double rb1 = R * rr1;
+    double rbn = rb1;
+    double cphi = rho1 * rr1;
+    double sphi = r.z * rr1;
+    double cm, sm;
+    if (rho2 < 2.2250738585072014e-308) { cm = 1; sm = 0; }
+    else { cm = r.x / rho1; sm = r.y / rho1; }
+    double pnm[O_GP_MAXN], pn1m[O_GP_MAXN];
+    pnm[0] = 0;
+    pn1m[0] = 1;
+    double ppn = 1, pn = 1, pn1 = sphi;
+    double zx = 0, zz = 0;
+    o_v3 t = v3(0, 0, 0);
+    for (int64_t n = 2; n <= gp->Nz; ++n) {
+        rbn *= rb1;
+        const double *JCSn = gp->c + GP_J(n);
+        pn = ((double)(2 * n - 1) * sphi * pn1 - (double)(n - 1) * pn) / (double)n;
+        ppn = sphi * ppn + (double)n * pn1;
+        double z1 = JCSn[0] * rbn;
+        zx += (double)(n + 1) * z1 * pn;
+        zz -= z1 * ppn;
+        { double sw = pn; pn = pn1; pn1 = sw; }
+        if (n > gp->Nt) continue;
+        o_v3 f = v3(0, 0, 0);
+        double m1 = (double)(2 * n - 1) * sphi;
+        double m2 = (double)(-n) * sphi;
+        double smm = 0, cmm = 0, smm1 = 0, cmm1 = 0;
+        for (int64_t m = 1; m <= n; ++m) {
+            if (m == n) {
+                pn1m[m - 1] = 0;
+                pnm[m - 1] = (double)(2 * n - 1) * cphi * pnm[m - 2];
+            } else
+                pnm[m - 1] = (m1 * pn1m[m - 1] - (double)(n + m - 1) * pnm[m - 1]) / (double)(n - m);
+            if (m == 1) { smm = sm; cmm = cm; smm1 = 0; cmm1 = 1; }
+            else { smm = 2 * cm * smm1 - smm; cmm = 2 * cm * cmm1 - cmm; }
+            double t1 = JCSn[2 * m] * smm + JCSn[2 * m - 1] * cmm;
+            double t2 = JCSn[2 * m] * cmm - JCSn[2 * m - 1] * smm;
+            double ppnm = m2 * pnm[m - 1] + (double)(n + m) * pn1m[m - 1];
+            f.x += t1 * pnm[m - 1];
+            f.y += (double)m * t2 * pnm[m - 1];
+            f.z += t1 * ppnm;
+            { double sw = cmm; cmm = cmm1; cmm1 = sw; }
+            { double sw = smm; smm = smm1; smm1 = sw; }
+            { double sw = pnm[m - 1]; pnm[m - 1] = pn1m[m - 1]; pn1m[m - 1] = sw; }
+        }
+        f.x *= (double)(n + 1);
+        t = vadd(t, vscale(rbn, f));
+    }
+    t.x = zx - cphi * t.x;
+    t.z += zz * cphi;
+    t = vscale(rr2, t);
+    double g1 = t.x * cphi - t.z * sphi;
+    out[0] = g1 * cm - t.y * sm;
+    out[1] = g1 * sm + t.y * cm;
+    out[2] = cphi * t.z + sphi * t.x;
+}
+
+/* ------------------------------------------------------------------------- */
+/* ring::sum, src/physics/ring.impl:2070-2196                                  */
+/* ------------------------------------------------------------------------- */
+static const float FZ_TABLE[] = {
+#include "../ephemeris_amd/tables/ring_fzcorr.inc"
+};
+#include "../ephemeris_amd/tables/ring_pade.inc"
+static const double P_XSERKK[][2] = EPH_PADE_XSERKK;
+static const double P_XSERKM[][2] = EPH_PADE_XSERKM;
+static const double P_XSER0[][2] = EPH_PADE_XSER0;
+static const double P_KSERKK[][2] = EPH_PADE_KSERKK;
+static const double P_KSERKM[][2] = EPH_PADE_KSERKM;
+static const double P_KSER0[][2] = EPH_PADE_KSER0;
+
+float o_fzcorr(float x, float y, int kh) {   /* ring.impl:2070-2089 */
+    const int N = 32;
+    x *= (float)N;
+    y *= (float)N;
+    int i = (int)floorf(x), j = (int)floorf(y);
+    j -= kh * N;
+    if (i < 0) i = 0;
+    if (j < 0) j = 0;
+    if (i >= N) i = N - 1;
+    if (j >= N) j = N - 1;
+    j += kh * N;
+    x -= (float)i;
+    y -= (float)j;
+    const float *p = FZ_TABLE + 16 * (2 * N * i + j);
+    return p[0] + x * (p[4] + x * (p[8] + x * p[12]))
+         + y * (p[1] + x * (p[5] + x * (p[9] + x * p[13]))
+         + y * (p[2] + x * (p[6] + x * (p[10] + x * p[14]))
+         + y * (p[3] + x * (p[7] + x * (p[11] + x * p[15])))));
+}
+static double padesum(double x, const double coef[][2], int n) {
+    double resn = 0, resd = 0;
+    while (n > 0) {
+        --n;
+        resn = x * resn + coef[n][0];
+        resd = x * resd + coef[n][1];
+    }
+    return resn / resd;
+}
+#define PADESUM(x, c) padesum((x), (c), (int)(sizeof(c) / sizeof((c)[0])))
+
+void o_ring_sum(const o_ring *rg, double host_R, const double *pr, double *out) {
+    const double pi = PI_REF;
+    o_v3 r = v3(pr[0], pr[1], pr[2]);
+    int psign = r.z < 0;
+    if (psign) r.z = -r.z;
+    const double re2 = r.x * r.x + r.y * r.y;
+    const double re = sqrt(re2);
+    const double rz2 = r.z * r.z;
+    const double r2 = re2 + rz2;
+    const double rz = r.z;
+    o_v3 ret = v3(0, 0, 0);
+    for (int64_t i = 0; i < rg->N; ++i) {
+        double R = rg->R_R[i] * host_R;
+        double H = rg->H_R[i] * host_R;
+        double reR = re * R;
+        double R2 = R * R, H2, fzh = 1;
+        double k2dre = 4 / (r2 + R2 + 2 * reR);
+        if (H > 0) {
+            H2 = H * H;
+            k2dre -= H2 * (k2dre * k2dre) * (1 - k2dre * rz2) / 48;
+            fzh = (re > R ? re - R : 0);
+            fzh = 4 * (rz2 + fzh * fzh);
+            fzh = sqrt(fzh / (fzh + H2));
+        }
+        double k2 = reR * k2dre;
+        k2dre *= R2 * sqrt(k2dre);
+        double phi = atan2(rz, re - R);
+        double k = 0, lk = 0, fx;
+        int lkh = k2 >= 0.45;
+        if (lkh) {
+            k = sqrt(k2);
+            lk = log((1 - k2) / (8 * (1 + k)));
+            fx = (PADESUM(k, P_XSERKK) + lk * PADESUM(k, P_XSERKM)) / (k2 * k2);
+        } else {
+            fx = PADESUM(k2, P_XSER0);
+        }
+        fx *= k2dre;
+        double fz;
+        double s = sin(phi), c = cos(phi);
+        int kh = k2 >= .5;
+        int exphi;
+        if (kh) {
+            const double phimax = (pi / 2);
+            exphi = c > 0;
+            double ck = 1 - k2, lk2 = log(ck / 16), sk = sqrt(ck);
+            double cc = exphi ? -c : c;
+            double cphi = exphi ? pi - phi : phi;
+            fz = cphi - (s / 8) * ((lk2 * (k2 - 5) - 2 * ck) * sk + 2 * cc * (1 + lk2) * ck);
+            fz *= o_fzcorr((float)((exphi ? phi : pi - phi) * (1 / phimax)), (float)(k2 * 2), kh);
+        } else {
+            const double phimax = (809. / 512);
+            exphi = phi > phimax;
+            double cr2 = exphi ? R2 + rz2 : r2;
+            double cRdr2 = (exphi ? re2 : R2) / cr2;
+            fz = (pi / 8) * (4 - 3 * cRdr2) * (rz / sqrt(cr2) * cRdr2);
+            fz *= o_fzcorr((float)((exphi ? pi - phi : phi) * (1 / phimax)), (float)(k2 * 2), kh);
+        }
+        if (exphi) {
+            double ek;
+            if (lkh) ek = PADESUM(k, P_KSERKK) + lk * PADESUM(k, P_KSERKM);
+            else ek = PADESUM(k2, P_KSER0);
+            fz = pi - 2 * sqrt(1 - k2) * ek * s - fz;
+        }
+        double Gs = rg->GsR2[i] / (host_R * host_R);
+        fx *= Gs;
+        fz *= Gs;
+        ret = vadd(ret, v3(-4 * fx * r.x, -4 * fx * r.y, -2 * fz * fzh));
+    }
+    if (psign) ret.z = -ret.z;
+    out[0] = ret.x; out[1] = ret.y; out[2] = ret.z;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Force macros of src/physics/mass.impl                                       */
+/* ------------------------------------------------------------------------- */
+typedef struct { double phi; o_v3 naccel; o_m3 C; } o_acc1;
+
+/* DAMPED_TIDAL_DEFORMATION_MATRIX, mass.impl:1-23 */
+INL void tidal_damped(const o_mass *mi, const o_mass *mj, o_acc1 *M) {
+    o_v3 r = ddvdiff(mj->r, mi->r);
+    o_v3 v = ddvdiff(mj->v, mi->v);
+    double rr2 = 1 / smax(vdot(r, r), mi->sR2 + mj->sR2);
+    double rr = sqrt(rr2);
+    double tp_dphi = rr * mj->GM;
+    M->phi -= tp_dphi;
+    double tp_dg = rr2 * tp_dphi;
+    M->naccel = vadd(M->naccel, vscale(tp_dg, r));
+    double fmj = -3 * tp_dg;
+    o_v3 dw = vscale(rr2, vcross(r, vsub(vcross(mi->w, r), v)));
+    double dwt = vnormalize(&dw);
+    dwt *= mi->tide_delay * mj->tide_delay_factor;
+    double ecwt2 = 1 + 4 * dwt * dwt,
+           secwt2 = sqrt(ecwt2),
+           x0 = sqrt((1 + secwt2) / (2 * ecwt2)),
+           y0 = dwt * sqrt(2 / (ecwt2 * (1 + secwt2))),
+           z02 = dwt * dwt * (2 / (ecwt2 + secwt2));
+    o_v3 dr = vadd(vscale(x0, r), vscale(y0, vcross(dw, r)));
+    o_m3 t = msub(mouter(vscale(rr2, dr), dr), mouter(vscale(z02, dw), dw));
+    t = madddiag(t, z02 - 1.0 / 3);
+    M->C = msub(M->C, mscale(fmj * mi->k2, t));
+}
+/* DAMPED_TIDAL_DEFORMATION_MATRIX_NDELAY, mass.impl:25-36 */
+INL void tidal_ndelay(const o_mass *mi, const o_mass *mj, o_acc1 *M) {
+    o_v3 r = ddvdiff(mj->r, mi->r);
+    double rr2 = 1 / smax(vdot(r, r), mi->sR2 + mj->sR2);
+    double rr = sqrt(rr2);
+    double tp_dphi = rr * mj->GM;
+    M->phi -= tp_dphi;
+    double tp_dg = rr2 * tp_dphi;
+    M->naccel = vadd(M->naccel, vscale(tp_dg, r));
+    double fmj = -3 * tp_dg;
+    o_m3 t = msubdiag(mouter(vscale(rr2, r), r), 1.0 / 3);
+    M->C = msub(M->C, mscale(fmj * mi->k2, t));
+}
+/* DAMPED_TIDAL_DEFORMATION_MATRIX_NK2, mass.impl:38-46 */
+INL void tidal_nk2(const o_mass *mi, const o_mass *mj, o_acc1 *M) {
+    o_v3 r = ddvdiff(mj->r, mi->r);
+    double rr2 = 1 / smax(vdot(r, r), mi->sR2 + mj->sR2);
+    double rr = sqrt(rr2);
+    double tp_dphi = rr * mj->GM;
+    double tp_dg = rr2 * tp_dphi;
+    M->phi -= tp_dphi;
+    M->naccel = vadd(M->naccel, vscale(tp_dg, r));
+}
+/* UPDATE_HARMONICS, mass.impl:48-61 */
+static void update_harmonics(o_mass *mi) {
+    double w2 = vdot(mi->w, mi->w);
+    mi->C_potential = madddiag(mi->C_potential, w2 * mi->k2r / 3);
+    mi->C_potential = msub(mi->C_potential, mouter(vscale(mi->k2r, mi->w), mi->w));
+    mi->C_potential = mscale(mi->R * mi->R2 / (2 * mi->GM), mi->C_potential);
+    o_m3 fmis = ddm2m(mi->s);
+    o_m3 mc = mi->C_static;
+    mc.x.x += mi->exJ2 / 2;
+    mc.y.y += mi->exJ2 / 2;
+    mc.z.z -= mi->exJ2;
+    mc = mtoworld(fmis, mc);
+    mi->C_potential = madd(mi->C_potential, mc);
+    mi->GI = mscale(-2.0 / 3 * mi->R2, msubdiag(mi->C_potential, mi->A));
+}
+/* UPDATE_ANGULAR_VELOCITY, mass.impl:63-72; returns should_break */
+static int update_angular_velocity(o_mass *mi) {
+    if (ddv_is_finite(mi->GL)) {
+        o_v3 oldw = mi->w;
+        mi->w = mmulv(minverse(mi->GI), ddv2v(mi->GL));
+        o_v3 d = vsub(mi->w, oldw);
+        return vdot(d, d) < 1e-18 * vdot(oldw, oldw);
+    }
+    mi->GL = ddv(mmulv(mi->GI, mi->w));
+    return 1;
+}
+/* mass::deform_by, src/physics/mass.cpp:78-103 */
+void o_deform_by(o_mass *ms, int64_t n, int64_t i) {
+    o_mass *mi = &ms[i];
+    o_acc1 M;
+    M.phi = 0;
+    M.naccel = v3(0, 0, 0);
+    M.C = mzero();
+    if (mi->k2 != 0) {
+        if (mi->tide_delay != 0) {
+            for (int64_t j = 0; j < n; ++j) if (j != i) tidal_damped(mi, &ms[j], &M);
+        } else {
+            for (int64_t j = 0; j < n; ++j) if (j != i) tidal_ndelay(mi, &ms[j], &M);
+        }
+    } else {
+        for (int64_t j = 0; j < n; ++j) if (j != i) tidal_nk2(mi, &ms[j], &M);
+    }
+    mi->phi = M.phi;
+    mi->naccel = M.naccel;
+    mi->C_potential = M.C;
+    update_harmonics(mi);
+}
+/* the per-body prologue of accel: spin fixed point, PREPARE_RELATIVITY and
+ * the reset of the main-force accumulators (mass.cpp:120-134, 311-326) */
+static void accel_deform_body(o_mass *ms, int64_t n, int64_t i) {
+    o_mass *mi = &ms[i];
+    int max_iter = 4;                        /* MAX_ANGULAR_VELOCITY_ITER */
+    do {
+        o_deform_by(ms, n, i);
+        if (update_angular_velocity(mi)) break;
+    } while (--max_iter);
+    mi->beta = vdivs(ddv2v(mi->v), C_LIGHT);       /* PREPARE_RELATIVITY, mass.impl:76-80 */
+    mi->beta2 = vdot(mi->beta, mi->beta);
+    mi->phi /= C2_LIGHT;
+    mi->naccel = vdivs(mi->naccel, C2_LIGHT);
+    mi->daccel = mi->dtorque = mi->gaccel = v3(0, 0, 0);
+}
+
+/* RELATIVITY, mass.impl:82-97; returns r, rr, rr2, rr3, tp_dg, b */
+typedef struct { o_v3 r, b; double rr2, rr, rr3, tp_dg; } o_pair;
+INL o_pair relativity(const o_mass *mi, const o_mass *mj, o_v3 *gaccel) {
+    o_pair p;
+    p.r = ddvdiff(mj->r, mi->r);
+    p.rr2 = 1 / smax(vdot(p.r, p.r), mi->sR2 + mj->sR2);
+    p.rr = sqrt(p.rr2);
+    p.rr3 = p.rr * p.rr2;
+    double tp_dphi = p.rr * mj->GM;
+    p.tp_dg = p.rr2 * tp_dphi;
+    double rbj = vdot(p.r, mj->beta);
+    double tp_rbjrr2 = rbj * p.rr;
+    tp_rbjrr2 *= tp_rbjrr2;
+    double delta1 = 4 * mi->phi + mj->phi + mi->beta2 + 2 * mj->beta2
+                  - 4 * vdot(mi->beta, mj->beta) + (vdot(p.r, mj->naccel) - 3 * tp_rbjrr2) / 2;
+    p.b = vsub(mj->beta, mi->beta);
+    o_v3 g = vadd(vadd(vscale(7 * tp_dphi / 2, mj->naccel), vscale(p.tp_dg * delta1, p.r)),
+                  vscale(p.tp_dg * (rbj - vdot(p.r, p.b) * 4), p.b));
+    *gaccel = vadd(*gaccel, g);
+    return p;
+}
+/* ROTATIONAL_TIDAL_DEFORMATION_NANTI_FORCE, mass.impl:118-125 */
+INL void tidal_force_nanti(const o_mass *mi, const o_mass *mj, const o_pair *p, o_v3 *daccel, o_v3 *dtorque) {
+    o_v3 r = p->r;
+    o_v3 Cr = mmulv(mi->C_potential, r);
+    o_v3 dg = vscale(mj->GM * p->rr3 * p->rr2 * mi->R2,
+                     vsub(vscale(vdot(r, Cr) * 5 * p->rr2, r), vadd(Cr, Cr)));
+    *daccel = vadd(*daccel, dg);
+    *dtorque = vadd(*dtorque, vcross(r, dg));
+    o_v3 Cjr = mmulv(mj->C_potential, r);
+    *daccel = vadd(*daccel, vscale(mj->GM * p->rr3 * p->rr2 * mj->R2,
+                                   vsub(vscale(vdot(r, Cjr) * 5 * p->rr2, r), vadd(Cjr, Cjr))));
+}
+/* ROTATIONAL_TIDAL_DEFORMATION (with reaction), mass.impl:111-116 */
+INL void tidal_force_pair(o_mass *mi, o_mass *mj, const o_pair *p) {
+    o_v3 r = p->r;
+    o_v3 Cr = mmulv(mi->C_potential, r);
+    o_v3 dg = vscale(p->rr3 * p->rr2 * mi->R2, vsub(vscale(vdot(r, Cr) * 5 * p->rr2, r), vadd(Cr, Cr)));
+    mi->daccel = vadd(mi->daccel, vscale(mj->GM, dg));
+    mj->daccel = vsub(mj->daccel, vscale(mi->GM, dg));
+    mi->dtorque = vadd(mi->dtorque, vscale(mj->GM, vcross(r, dg)));
+}
+/* LENSE_THIRRING, mass.impl:99-106 */
+INL void lense_thirring(const o_mass *mi, const o_mass *mj, const o_pair *p, o_v3 *daccel, o_v3 *dtorque) {
+    double rcr32 = 2 * mj->GM / C_LIGHT * p->rr3;
+    o_v3 GL = ddv2v(mi->GL);
+    o_v3 GLb = vcross(GL, p->b);
+    *daccel = vsub(*daccel, vscale(rcr32, vsub(GLb, vscale(3 * vdot(GLb, p->r) * p->rr2, p->r))));
+    *dtorque = vsub(*dtorque, vscale(rcr32, vcross(GL, vcross(p->r, p->b))));
+    o_v3 GLj = ddv2v(mj->GL);
+    *daccel = vadd(*daccel, vcross(vscale(rcr32, vsub(GLj, vscale(3 * vdot(GLj, p->r) * p->rr2, p->r))), p->b));
+}
+/* RADIATION_PRESSURE, mass.impl:108-109 */
+INL void radiation(const o_mass *mi, const o_mass *mj, const o_pair *p, o_v3 *daccel) {
+    *daccel = vadd(*daccel, vscale(mj->lum * mi->rR2G_4c * p->rr3 / mi->GM, p->r));
+}
+/* the acceleration of a point near a non-point body, in the world frame:
+ * geopotential (mass.cpp:186-188) or ring (mass.cpp:200-203) */
+static o_v3 nonpoint_field(const o_system *s, const o_mass *mi, int is_gp, o_v3 r) {
+    double lr[3], an[3];
+    if (is_gp) {
+        o_m3 fmis = ddm2m(mi->s);
+        o_v3 l = vmulm(r, fmis);
+        lr[0] = l.x; lr[1] = l.y; lr[2] = l.z;
+        o_gp_sum(&s->gp[mi->gp], mi->R, lr, an);
+        return mmulv(fmis, v3(an[0], an[1], an[2]));
+    }
+    o_v3 migl = ddv2v(mi->GL);
+    o_v3 x = vasc_node(migl), z = vunit(migl);
+    o_m3 fgls = m3c(x, vcross(z, x), z);
+    o_v3 l = vmulm(r, fgls);
+    lr[0] = l.x; lr[1] = l.y; lr[2] = l.z;
+    o_ring_sum(&s->ring[mi->ring], mi->R, lr, an);
+    return mmulv(fgls, v3(an[0], an[1], an[2]));
+}
+
+/* ------------------------------------------------------------------------- */
+/* msystem::accel, src/physics/mass.cpp:216-397                                */
+/* ------------------------------------------------------------------------- */
+static void accel_mainforce_body(o_mass *ms, int64_t n, int64_t i) { /* mass.cpp:136-157 */
+    o_mass *mi = &ms[i];
+    for (int64_t j = 0; j < n; ++j) {
+        if (i == j) continue;
+        const o_mass *mj = &ms[j];
+        o_pair p = relativity(mi, mj, &mi->gaccel);
+        checked_maximize(&mi->min_distance, p.rr);
+        checked_maximize(&mi->max_influence, p.tp_dg);
+        tidal_force_nanti(mi, mj, &p, &mi->daccel, &mi->dtorque);
+        lense_thirring(mi, mj, &p, &mi->daccel, &mi->dtorque);
+        radiation(mi, mj, &p, &mi->daccel);
+    }
+}
+
+typedef struct { int64_t i; int is_gp; } o_item;
+
+/* non-point rounds of the pooled path, mass.cpp:253-307 with the task body
+ * accel_nonpoint, mass.cpp:158-214 */
+static void accel_nonpoint_pooled(o_system *s, int64_t n_task) {
+    o_mass *ms = s->m;
+    int64_t mn = s->n;
+    int64_t n_nonpoint = 0;
+    for (int64_t i = 0; i < mn; ++i) {
+        o_mass *mi = &ms[i];
+        if (mi->gp >= 0) { ++n_nonpoint; mi->idaccel = mi->idtorque = v3(0, 0, 0); }
+        if (mi->ring >= 0) { ++n_nonpoint; mi->jdaccel = mi->jdtorque = v3(0, 0, 0); }
+    }
+    int64_t T = n_nonpoint < n_task ? n_nonpoint : n_task;
+    if (T > 0) {
+        /* items in creation order: bodies from the last to the first, the
+         * geopotential item before the ring item of the same body */
+        o_item *items = (o_item *)malloc(sizeof(o_item) * (size_t)n_nonpoint);
+        int64_t c = 0;
+        for (int64_t i = mn; i > 0;) {
+            --i;
+            if (ms[i].gp >= 0) { items[c].i = i; items[c].is_gp = 1; ++c; }
+            if (ms[i].ring >= 0) { items[c].i = i; items[c].is_gp = 0; ++c; }
+        }
+        for (int64_t k = 0; k < T; ++k) {
+            for (int64_t ch = 0; ch < T; ++ch) {
+                int64_t first = ch * n_nonpoint / T, last = (ch + 1) * n_nonpoint / T;
+                int64_t kk = ch + k;
+                if (kk >= T) kk -= T;
+                int64_t j_start = kk * mn / T, j_end = (kk + 1) * mn / T;
+                /* the chain runs from its last-created item back to its first */
+                for (int64_t it = last - 1; it >= first; --it) {
+                    int64_t i = items[it].i;
+                    o_mass *mi = &ms[i];
+                    o_v3 td = v3(0, 0, 0), tq = v3(0, 0, 0);
+                    for (int64_t j = j_start; j < j_end; ++j) {
+                        if (i == j) continue;
+                        o_mass *mj = &ms[j];
+                        o_v3 r = ddvdiff(mj->r, mi->r);
+                        o_v3 an = nonpoint_field(s, mi, items[it].is_gp, r);
+                        mj->daccel = vadd(mj->daccel, vscale(mi->GM, an));   /* APPLY_NONPOINT_FORCE */
+                        td = vsub(td, vscale(mj->GM, an));
+                        tq = vsub(tq, vscale(mj->GM, vcross(r, an)));
+                    }
+                    if (items[it].is_gp) {
+                        mi->idaccel = vadd(mi->idaccel, td);
+                        mi->idtorque = vadd(mi->idtorque, tq);
+                    } else {
+                        mi->jdaccel = vadd(mi->jdaccel, td);
+                        mi->jdtorque = vadd(mi->jdtorque, tq);
+                    }
+                }
+            }
+        }
+        free(items);
+    }
+    if (n_nonpoint) for (int64_t i = 0; i < mn; ++i) {
+        o_mass *mi = &ms[i];
+        if (mi->gp >= 0) { mi->daccel = vadd(mi->daccel, mi->idaccel); mi->dtorque = vadd(mi->dtorque, mi->idtorque); }
+        if (mi->ring >= 0) { mi->daccel = vadd(mi->daccel, mi->jdaccel); mi->dtorque = vadd(mi->dtorque, mi->jdtorque); }
+    }
+}
+
+void o_accel(o_system *s, int parallel_option, int pool) {
+    o_mass *ms = s->m;
+    int64_t mn = s->n;
+    const int64_t n_tasklimit = 64, n_tasksize = 48 * 48;
+    int64_t sq = (mn * mn + (n_tasksize - 1)) / n_tasksize;
+    int64_t n_task = (int64_t)parallel_option | (mn < sq ? mn : sq);
+    if (n_task > 1 && pool) {
+        if (n_task > n_tasklimit) n_task = n_tasklimit;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(s->nthreads)
+        for (int64_t i = 0; i < mn; ++i) accel_deform_body(ms, mn, i);
+#pragma omp parallel for schedule(dynamic, 4) num_threads(s->nthreads)
+        for (int64_t i = 0; i < mn; ++i) accel_mainforce_body(ms, mn, i);
+        accel_nonpoint_pooled(s, n_task);
+    } else {
+        for (int64_t i = 0; i < mn; ++i) accel_deform_body(ms, mn, i);
+        for (int64_t i = 0; i < mn; ++i) {
+            o_mass *mi = &ms[i];
+            for (int64_t j = 0; j < mn; ++j) {
+                if (i == j) continue;
+                o_mass *mj = &ms[j];
+                o_pair p = relativity(mi, mj, &mi->gaccel);
+                checked_maximize(&mi->min_distance, p.rr);
+                checked_maximize(&mi->max_influence, p.tp_dg);
+                tidal_force_pair(mi, mj, &p);
+                lense_thirring(mi, mj, &p, &mi->daccel, &mi->dtorque);
+                radiation(mi, mj, &p, &mi->daccel);
+                if (mi->gp >= 0) {
+                    o_v3 an = nonpoint_field(s, mi, 1, p.r);
+                    mj->daccel = vadd(mj->daccel, vscale(mi->GM, an));
+                    mi->daccel = vsub(mi->daccel, vscale(mj->GM, an));
+                    mi->dtorque = vsub(mi->dtorque, vscale(mj->GM, vcross(p.r, an)));
+                }
+                if (mi->ring >= 0) {
+                    o_v3 an = nonpoint_field(s, mi, 0, p.r);
+                    mj->daccel = vadd(mj->daccel, vscale(mi->GM, an));
+                    mi->daccel = vsub(mi->daccel, vscale(mj->GM, an));
+                    mi->dtorque = vsub(mi->dtorque, vscale(mj->GM, vcross(p.r, an)));
+                }
+            }
+        }
+    }
+    /* tidal-matrix correction (mass.cpp:376-387) applies only to children of
+     * a combined-integration subsystem; such lists are never built here. */
+    for (int64_t i = 0; i < mn; ++i) {
+        o_mass *mi = &ms[i];
+        mi->phi *= C2_LIGHT;                     /* FINALIZE_RELATIVITY */
+        mi->naccel = vscale(C2_LIGHT, mi->naccel);
+        if (mi->ring >= 0) {                     /* RING_CORRECTION, mass.impl:143-152 */
+            const o_ring *mr = &s->ring[mi->ring];
+            mi->daccel = vsub(mi->daccel, vscale(mr->GM_ratio, vadd(vadd(mi->gaccel, mi->daccel), mi->naccel)));
+            o_v3 mGL = ddv2v(mi->GL);
+            double rmGL = vnormalize(&mGL);
+            o_v3 ptorque = vscale(vdot(mi->dtorque, mGL), mGL);
+            o_v3 dtorque = vsub(mi->dtorque, ptorque);
+            dtorque = vscale(rmGL / (rmGL + mr->GL_R2 * mi->R2), dtorque);
+            mi->dtorque = vadd(dtorque, ptorque);
+        }
+    }
+}
+
+void o_deform(o_system *s) {
+    for (int64_t i = 0; i < s->n; ++i) o_deform_by(s->m, s->n, i);
+}
+
+void o_update(o_system *s, double t) {
+    for (int64_t i = 0; i < s->n; ++i) {
+        o_mass *mi = &s->m[i];
+        mi->GM = mi->GM0 + mi->dGM * t;
+        mi->exJ2 = mi->dJ2 * t;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* RK12, src/integrators/RungeKutta.impl:107-194 (CPU macros :20-35)           */
+/* ------------------------------------------------------------------------- */
+static const double RK12_TABLE[325][2] = {
+#include "../ephemeris_amd/tables/rk12_feagin.inc"
+};
+const double *o_rk12_table(void) { return &RK12_TABLE[0][0]; }
+
+typedef struct { o_ddv3 r, v, GL; o_ddm3 s; o_v3 w; } o_state0;
+typedef struct { o_ddv3 v; o_v3 a, w, dtorque; } o_stage;
+
+static void orthogonalize(o_mass *m) {       /* mass.h:147-150 */
+    o_m3 f = ddm2m(m->s);
+    o_m3 t = madddiag(mneg(mmulm(mtranspose(f), f)), 1);
+    m->s = ddmadd(m->s, mmulm(mscale(0.5, f), t));
+}
+
+static void rk12(o_system *s, double dt, int64_t n_step, int pool) {
+    int64_t mn = s->n;
+    o_mass *x = s->m;
+    o_state0 *x0 = (o_state0 *)malloc(sizeof(o_state0) * (size_t)mn);
+    o_stage *f = (o_stage *)malloc(sizeof(o_stage) * (size_t)(25 * mn));
+    for (int64_t i_step = 0; i_step < n_step; ++i_step) {
+        for (int64_t i = 0; i < mn; ++i) {
+            x0[i].r = x[i].r; x0[i].v = x[i].v; x0[i].GL = x[i].GL; x0[i].s = x[i].s; x0[i].w = x[i].w;
+            x[i].Egrad = x[i].dtorque;
+        }
+        double dt_k;
+        for (int k = 1; k <= 25; ++k) {
+            dt_k = 0;
+            for (int64_t i = 0; i < mn; ++i) {
+                o_stage *fi = &f[mn * (k - 1) + i];
+                o_mass *xi = &x[i];
+                fi->v = xi->v;
+                fi->a = vadd(vadd(xi->gaccel, xi->daccel), xi->naccel);
+                fi->w = xi->w;
+                fi->dtorque = xi->dtorque;
+                if (k == 25) {
+                    o_m3 ir = minverse(xi->GI);
+                    double irdet = mdet(ir);
+                    double dt2 = dt * dt;
+                    o_v3 tq = vscale(dt2, xi->dtorque);
+                    o_v3 tqdot = vscale(dt2, vsub(xi->dtorque, xi->Egrad));
+                    o_v3 j = ddv2v(xi->GL), w = vscale(dt, xi->w);
+                    o_v3 irtq = mmulv(ir, tq);
+                    j = vscale(dt, j);
+                    o_v3 t1 = vscale(vdot(w, j), vadd(mmulv(ir, w), vscale(irdet, mmulv(xi->GI, j))));
+                    o_v3 t2 = vcross(w, irtq);
+                    o_v3 t3 = mmulv(ir, vadd(vsub(vcross(j, irtq), vscale(2, vcross(w, tq))), tqdot));
+                    xi->Egrad = vcross(vadd(vadd(t1, t2), t3), w);
+                }
+                xi->v = ddv(v3(0, 0, 0));
+                xi->naccel = v3(0, 0, 0);
+                xi->w = v3(0, 0, 0);
+                xi->dtorque = v3(0, 0, 0);
+            }
+            for (int j = 0; j < k; ++j) {
+                const double *c = RK12_TABLE[k * (k - 1) / 2 + j];
+                o_dd ckj = {c[0], c[1]};
+                double fckj = dd2d(ckj);
+                if (ckj.hi != 0) {
+                    dt_k += fckj;
+                    for (int64_t i = 0; i < mn; ++i) {
+                        const o_stage *fi = &f[mn * j + i];
+                        o_mass *xi = &x[i];
+                        xi->v = ddvadd(xi->v, ddvscale(ckj, fi->v));
+                        xi->naccel = vadd(xi->naccel, vscale(fckj, fi->a));
+                        xi->w = vadd(xi->w, vscale(fckj, fi->w));
+                        xi->dtorque = vadd(xi->dtorque, vscale(fckj, fi->dtorque));
+                    }
+                }
+            }
+            double t = dd2d(s->t_eph) + dt * (dt_k + (double)i_step);
+            for (int64_t i = 0; i < mn; ++i) {
+                const o_state0 *xi0 = &x0[i];
+                o_mass *xi = &x[i];
+                xi->r = ddvadd(xi0->r, ddvscale(dd(dt), xi->v));
+                xi->v = ddvadd(xi0->v, ddv(vscale(dt, xi->naccel)));
+                xi->w = vdivs(xi->w, dt_k);
+                xi->s = xi0->s;
+                o_v3 dw = vscale(2.0 / 3, vsub(xi->w, xi0->w));
+                xi->s = ddmadd(xi->s, mmulm(rotation_matrix(vsub(xi->w, dw), dt * dt_k / 2), ddm2m(xi->s)));
+                xi->s = ddmadd(xi->s, mmulm(rotation_matrix(vadd(xi->w, dw), dt * dt_k / 2), ddm2m(xi->s)));
+                xi->GL = ddvadd(xi0->GL, ddv(vscale(dt, xi->dtorque)));
+                if (k == 25) {
+                    xi->s = ddmadd(xi->s, mmulm(rotation_matrix(xi->Egrad, 1.0 / 72), ddm2m(xi->s)));
+                    orthogonalize(xi);
+                }
+                xi->GM = xi->GM0 + xi->dGM * t;
+                xi->exJ2 = xi->dJ2 * t;
+            }
+            o_accel(s, 0, pool);
+        }
+    }
+    free(x0);
+    free(f);
+}
+
+/* msystem::integrate, src/physics/mass.cpp:399-442 */
+void o_integrate(o_system *s, double dt, int64_t n_step, int pool) {
+    if (n_step == 0) return;
+    if (n_step < 0) { n_step = -n_step; dt = -dt; }
+    for (int64_t i = 0; i < s->n; ++i) s->m[i].max_influence = s->m[i].min_distance = 0;
+    rk12(s, dt, n_step, pool);
+    for (int64_t i = 0; i < s->n; ++i) s->m[i].min_distance = 1 / s->m[i].min_distance;
+    s->t_eph = dd_add(s->t_eph, dd(dt * (double)n_step));
+}
+
+/* ------------------------------------------------------------------------- */
+/* exported single functions                                                  */
+/* ------------------------------------------------------------------------- */
+void o_dd_ops(const double *a, const double *b, double *out) {
+    o_dd x = {a[0], a[1]}, y = {b[0], b[1]};
+    o_dd r0 = dd_add(x, y), r1 = dd_sub(x, y), r2 = dd_mul(x, y), r3 = dd_div(x, y);
+    out[0] = r0.hi; out[1] = r0.lo; out[2] = r1.hi; out[3] = r1.lo;
+    out[4] = r2.hi; out[5] = r2.lo; out[6] = r3.hi; out[7] = r3.lo;
+}
+void o_dd_sqrt(const double *a, double *out) {
+    o_dd x = {a[0], a[1]};
+    o_dd r = dd_sqrt(x);
+    out[0] = r.hi; out[1] = r.lo;
+}
+void o_rotation_matrix(const double *w, double t, double *out) {
+    o_m3 m = rotation_matrix(v3(w[0], w[1], w[2]), t);
+    memcpy(out, &m, sizeof m);
+}
+void o_inverse(const double *m9, double *out) {
+    o_m3 m;
+    memcpy(&m, m9, sizeof m);
+    o_m3 r = minverse(m);
+    memcpy(out, &r, sizeof r);
+}
+double o_normalize(double *a) { return normalize3(&a[0], &a[1], &a[2]); }
+void o_asc_node(const double *a, double *out) {
+    o_v3 r = vasc_node(v3(a[0], a[1], a[2]));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+/* ------------------------------------------------------------------------- */
+/* loader, src/modules/mass_loader.cpp:67-294, 324-561;                        */
+/* geopotential.cpp:8-127; ring.cpp:25-108                                    */
+/* ------------------------------------------------------------------------- */
+static char g_err[512];
+const char *o_load_error(void) { return g_err; }
+
+/* MFILE::readline, src/utils/memio.cpp:257-301: '\r' and '\n' end lines,
+ * empty lines and lines starting with '#' are skipped. */
+static int read_line(FILE *f, char *buf, size_t cap) {
+    size_t n = 0;
+    int c;
+    for (;;) {
+        n = 0;
+        int got = 0;
+        while ((c = fgetc(f)) != EOF) {
+            got = 1;
+            if (c == '\r') c = '\n';
+            if (c == '\n') break;
+            if (n + 1 < cap) buf[n++] = (char)c;
+        }
+        buf[n] = 0;
+        if (n > 0 && buf[0] == '#') n = 0, buf[0] = 0;
+        if (n > 0) return 1;
+        if (!got || c == EOF) return 0;
+    }
+}
+
+static int load_gp(const char *path, double factor, o_gp *out) {
+    FILE *f = fopen(path, "rb");
+    if (!f) { snprintf(g_err, sizeof g_err, "%s: cannot open geopotential", path); return 0; }
+    char line[1100];
+    long long Nz = -1, Nt = 0;
+    int64_t Nzmax = 0, Ntmax = 0;
+    const int64_t N_start = 3;
+    int ok = 0;
+    memset(out, 0, sizeof *out);
+    while (read_line(f, line, sizeof line)) {
+        if (Nz < 0) {
+            if (sscanf(line, "%lld%lld", &Nz, &Nt) != 2) { snprintf(g_err, sizeof g_err, "%s: bad header", path); break; }
+            if (Nt < 0) Nt = 0;
+            if (Nz < Nt) Nz = Nt;
+            if (Nz > O_GP_MAXN) { Nz = O_GP_MAXN; if (Nt > O_GP_MAXN) Nt = O_GP_MAXN; }
+            if (Nz < N_start) { snprintf(g_err, sizeof g_err, "%s: degree too low", path); break; }
+            out->Nz = Nz; out->Nt = Nt;
+            ok = 1;
+            continue;
+        }
+        long long n = 0, m = 0;
+        double Cnm = 0, Snm = 0;
+        int nr = sscanf(line, "%lld%lld%lf%lf", &n, &m, &Cnm, &Snm);
+        if (nr < 3 || n < 2 || m < 0 || m > n || (nr == 3 && m != 0) || (m == 0 && Snm != 0)) {
+            snprintf(g_err, sizeof g_err, "%s: bad line %s", path, line);
+            ok = 0;
+            break;
+        }
+        if (n < N_start) continue;
+        if (n > Nz || (n > Nt && m > 0)) continue;
+        double rr = pow(factor, (double)n);
+        Cnm *= rr;
+        Snm *= rr;
+        if (m == 0) out->c[GP_J(n)] += Cnm;
+        else {
+            out->c[GP_J(n) + 2 * m - 1] += Cnm;
+            out->c[GP_J(n) + 2 * m] += Snm;
+            if (n > Ntmax) Ntmax = n;
+        }
+        if (n > Nzmax) Nzmax = n;
+    }
+    fclose(f);
+    if (!ok) return 0;
+    if (!Nzmax) return -1;                   /* no valid terms: model ignored */
+    if (Nzmax < out->Nz) out->Nz = Nzmax;
+    if (Ntmax < out->Nt) out->Nt = Ntmax;
+    return 1;
+}
+
+static int load_ring(const char *path, double ref_GM, double ref_R, double dmf, o_ring *out) {
+    FILE *f = fopen(path, "rb");
+    if (!f) { snprintf(g_err, sizeof g_err, "%s: cannot open ring", path); return 0; }
+    char line[1100], name[1100];
+    double Gs_[O_RING_MAXDISK], R_[O_RING_MAXDISK], H_[O_RING_MAXDISK];
+    int nd = 0, ok = 1;
+    while (read_line(f, line, sizeof line)) {
+        double rin, rout, thick, GM;
+        if (sscanf(line, "%[^\t]%lf%lf%lf%lf", name, &rin, &rout, &thick, &GM) != 5) { ok = 0; break; }
+        GM *= fabs(dmf);
+        rin *= 1000; rout *= 1000; thick *= 1000;
+        double Gs = GM / (PI_REF * (rout - rin) * (rout + rin));
+        double gsr[2] = {-Gs, Gs}, rr[2] = {rin, rout};
+        for (int q = 0; q < 2; ++q) {          /* adddisk, ring.cpp:15-23 */
+            int found = 0;
+            for (int d = 0; d < nd; ++d)
+                if (R_[d] == rr[q] && H_[d] == thick) { Gs_[d] += gsr[q]; found = 1; break; }
+            if (!found && nd < O_RING_MAXDISK) { Gs_[nd] = gsr[q]; R_[nd] = rr[q]; H_[nd] = thick; ++nd; }
+        }
+    }
+    fclose(f);
+    if (!ok) { snprintf(g_err, sizeof g_err, "%s: bad ring line", path); return 0; }
+    memset(out, 0, sizeof *out);
+    out->N = nd;
+    double ref_R2 = ref_R * ref_R, ref_GMR2 = ref_GM * ref_R2;
+    for (int i = 0; i < nd; ++i) {
+        double R2 = R_[i] * R_[i], H2 = H_[i] * H_[i];
+        double GM = PI_REF * R2 * Gs_[i];
+        out->GM_ratio += GM;
+        out->A += GM * (6 * R2 + H2) / (12 * ref_GMR2);
+        out->J2 += GM * (3 * R2 - H2) / (12 * ref_GMR2);
+        out->GL_R2 += 4 * GM * sqrt(ref_GM * R_[i]) / (5 * ref_GMR2);
+        out->GsR2[i] = Gs_[i] / ref_GM * ref_R2;
+        out->R_R[i] = R_[i] / ref_R;
+        out->H_R[i] = H_[i] / ref_R;
+    }
+    out->GM_ratio = 1 / (1 + ref_GM / out->GM_ratio);
+    if (dmf < 0) out->GL_R2 *= -1;
+    return 1;
+}
+
+static o_system *sys_new(void) {
+    o_system *s = (o_system *)calloc(1, sizeof(o_system));
+    s->nthreads = 1;
+    s->delta_t = 300;
+    return s;
+}
+
+o_system *o_load_files(const char *initial, const char *extra, const char *gp_dir,
+                       const char *ring_dir, int pool_at_load) {
+    g_err[0] = 0;
+    FILE *f = fopen(initial, "rb");
+    if (!f) { snprintf(g_err, sizeof g_err, "%s: cannot open", initial); return NULL; }
+    o_system *s = sys_new();
+    int64_t cap = 64;
+    s->m = (o_mass *)calloc((size_t)cap, sizeof(o_mass));
+    char line[1100], sname[1100], sid[1100], path[2300];
+    int failed = 0;
+    while (read_line(f, line, sizeof line)) {
+        o_mass m;
+        memset(&m, 0, sizeof m);
+        double j2, c21, c22, s21, s22, ra, dec, W, p, gprf, ringmf;
+        o_v3 r, v, x, z, w;
+        int n = sscanf(line,
+            "%[^\t]%s%lf%lf" "%lf%lf%lf%lf" "%lf%lf%lf%lf%lf" "%lf%lf%lf"
+            "%lf%lf%lf%lf%lf%lf" "%lf%lf%lf%lf" "%lf%lf%lf%lf%lf%lf%lf%lf%lf",
+            sname, sid, &m.GM, &m.R, &m.inertia, &m.k2, &m.k2r, &m.tide_delay,
+            &j2, &c21, &c22, &s21, &s22, &m.tide_delay_factor, &gprf, &ringmf,
+            &r.x, &r.y, &r.z, &v.x, &v.y, &v.z, &ra, &dec, &W, &p,
+            &x.x, &x.y, &x.z, &z.x, &z.y, &z.z, &w.x, &w.y, &w.z);
+        if (!(n == 35 || n == 26)) { snprintf(g_err, sizeof g_err, "line length error: %s", line); failed = 1; break; }
+        size_t sl = strlen(sid);
+        if (sl >= 8) { snprintf(g_err, sizeof g_err, "sid too long: %s", sid); failed = 1; break; }
+        m.sid = 0;
+        memcpy(&m.sid, sid, sl);
+        for (int64_t q = 0; q < s->n; ++q)
+            if (s->m[q].sid == m.sid) { snprintf(g_err, sizeof g_err, "duplicate sid %s", sid); failed = 1; }
+        if (failed) break;
+        m.R *= 1000;
+        m.GM0 = m.GM;
+        m.dGM = m.lum = m.recpt = m.exJ2 = m.dJ2 = 0;
+        m.gp = m.ring = -1;
+        if (gprf != 0) {
+            snprintf(path, sizeof path, "%s/%s.txt", gp_dir ? gp_dir : ".", sid);
+            o_gp g;
+            int rc = load_gp(path, gprf, &g);
+            if (rc == 0) { failed = 1; break; }
+            if (rc < 0) { failed = 1; snprintf(g_err, sizeof g_err, "%s: no valid terms", path); break; }
+            s->gp = (o_gp *)realloc(s->gp, sizeof(o_gp) * (size_t)(s->n_gp + 1));
+            s->gp[s->n_gp] = g;
+            m.gp = s->n_gp++;
+        }
+        if (ringmf != 0) {
+            snprintf(path, sizeof path, "%s/%s.txt", ring_dir ? ring_dir : ".", sid);
+            o_ring g;
+            if (!load_ring(path, m.GM, m.R, ringmf, &g)) { failed = 1; break; }
+            if (g.N > 0) {
+                s->ring = (o_ring *)realloc(s->ring, sizeof(o_ring) * (size_t)(s->n_ring + 1));
+                s->ring[s->n_ring] = g;
+                m.ring = s->n_ring++;
+            }
+        }
+        if (n == 26) {                            /* mass_loader.cpp:441-451 */
+            o_v3 tz, tx;
+            double th = (90 - dec) * DEGREE, ph = ra * DEGREE, cth = sin(th);
+            tz = v3(cth * cos(ph), cth * sin(ph), cos(th));
+            th = 90 * DEGREE; ph = (ra + 90) * DEGREE; cth = sin(th);
+            tx = v3(cth * cos(ph), cth * sin(ph), cos(th));
+            tx = vadd(tx, mmulv(rotation_matrix(tz, W * DEGREE), tx));
+            const double ob = 84381.448 / 3600 * DEGREE;
+            { double sb = sin(-ob), cb = cos(-ob), ny = tx.y * cb - tx.z * sb; tx.z = tx.z * cb + tx.y * sb; tx.y = ny; }
+            { double sb = sin(-ob), cb = cos(-ob), ny = tz.y * cb - tz.z * sb; tz.z = tz.z * cb + tz.y * sb; tz.y = ny; }
+            x = tx; z = tz;
+            w = vscale(2 * PI_REF / (p * 3600), z);
+        }
+        m.s.x = ddv(x);
+        m.s.z = ddv(z);
+        m.s.y = ddv(vcross(z, x));
+        orthogonalize(&m);
+        m.r = ddv(r);
+        m.v = ddv(v);
+        m.w = w;
+        m.A = 3 * m.inertia / 2;
+        m.R2 = m.R * m.R;
+        m.sR2 = m.R < 0 ? m.R2 : 0;
+        m.rR2G_4c = 0;
+        m.C_static = from_harmonics(j2, c21, c22, s21, s22);
+        o_m3 fmis = ddm2m(m.s);
+        m.GI = mscale(-2.0 / 3 * m.R2, msubdiag(mtoworld(fmis, m.C_static), m.A));
+        m.GL = ddv(mmulv(m.GI, m.w));
+        if (s->n == cap) { cap *= 2; s->m = (o_mass *)realloc(s->m, sizeof(o_mass) * (size_t)cap); }
+        s->m[s->n++] = m;
+    }
+    fclose(f);
+    if (!failed && extra && *extra) {
+        FILE *fe = fopen(extra, "rb");
+        if (!fe) { snprintf(g_err, sizeof g_err, "%s: cannot open", extra); failed = 1; }
+        else {
+            while (read_line(fe, line, sizeof line)) {
+                double param;
+                if (sscanf(line, "%s%s%lf", sid, sname, &param) != 3) { failed = 1; snprintf(g_err, sizeof g_err, "extra line error"); break; }
+                uint64_t key = 0;
+                size_t sl = strlen(sid);
+                if (sl < 8) memcpy(&key, sid, sl);
+                o_mass *m = NULL;
+                for (int64_t q = 0; q < s->n; ++q) if (s->m[q].sid == key) m = &s->m[q];
+                if (!m) continue;
+                if (!strcmp(sname, "MassRate")) m->dGM = param;
+                else if (!strcmp(sname, "Luminosity")) m->lum = param;
+                else if (!strcmp(sname, "Receptance")) { m->recpt = param; m->rR2G_4c = m->recpt * m->R2 * (G_NEWTON / (4 * C_LIGHT)); }
+                else if (!strcmp(sname, "J2Rate")) m->dJ2 = param;
+                else { failed = 1; snprintf(g_err, sizeof g_err, "unknown extra %s", sname); break; }
+            }
+            fclose(fe);
+        }
+    }
+    if (failed) { o_free(s); return NULL; }
+    o_deform(s);                                  /* mass_loader.cpp:540-551 */
+    for (int64_t i = 0; i < s->n; ++i) {
+        o_mass *m = &s->m[i];
+        o_m3 fmis = ddm2m(m->s);
+        o_m3 ncp = mtoworld(fmis, m->C_static);
+        m->C_static = msub(m->C_static, msub(mtolocal(fmis, m->C_potential), m->C_static));
+        m->C_potential = ncp;
+    }
+    o_accel(s, 0, pool_at_load);
+    return s;
+}
+
+static void join_path(char *out, size_t cap, const char *dir, const char *name) {
+    snprintf(out, cap, "%s/%s", dir, name);
+}
+
+o_system *o_load_config(const char *config_path, int pool_at_load) {
+    g_err[0] = 0;
+    FILE *f = fopen(config_path, "rb");
+    if (!f) { snprintf(g_err, sizeof g_err, "%s: cannot open", config_path); return NULL; }
+    char dir[1100];
+    strncpy(dir, config_path, sizeof dir - 1);
+    dir[sizeof dir - 1] = 0;
+    char *slash = strrchr(dir, '/');
+    if (slash) *slash = 0; else strcpy(dir, ".");
+    char line[1100], key[1100], val[1100];
+    char initial[1100] = "", extra[1100] = "", gpd[1100] = "", rgd[1100] = "";
+    double delta_t = 300;
+    long long tdb = 0;
+    while (read_line(f, line, sizeof line)) {
+        if (sscanf(line, "%s%s", key, val) != 2) continue;
+        if (!strcmp(key, "Initial")) strcpy(initial, val);
+        else if (!strcmp(key, "Extra")) strcpy(extra, val);
+        else if (!strcmp(key, "Geopotentials")) strcpy(gpd, val);
+        else if (!strcmp(key, "Rings")) strcpy(rgd, val);
+        else if (!strcmp(key, "Delta_t")) delta_t = atof(val);
+        else if (!strcmp(key, "TDB")) tdb = atoll(val);
+    }
+    fclose(f);
+    char p1[2300], p2[2300], p3[2300], p4[2300];
+    join_path(p1, sizeof p1, dir, initial);
+    join_path(p2, sizeof p2, dir, extra);
+    join_path(p3, sizeof p3, dir, gpd);
+    join_path(p4, sizeof p4, dir, rgd);
+    o_system *s = o_load_files(p1, extra[0] ? p2 : NULL, gpd[0] ? p3 : NULL, rgd[0] ? p4 : NULL, pool_at_load);
+    if (s) { s->delta_t = delta_t; s->t_eph = dd((double)tdb); }
+    return s;
+}
+
+o_system *o_clone(const o_system *src) {
+    o_system *s = sys_new();
+    *s = *src;
+    s->m = (o_mass *)malloc(sizeof(o_mass) * (size_t)(src->n ? src->n : 1));
+    memcpy(s->m, src->m, sizeof(o_mass) * (size_t)src->n);
+    s->gp = (o_gp *)malloc(sizeof(o_gp) * (size_t)(src->n_gp ? src->n_gp : 1));
+    memcpy(s->gp, src->gp, sizeof(o_gp) * (size_t)src->n_gp);
+    s->ring = (o_ring *)malloc(sizeof(o_ring) * (size_t)(src->n_ring ? src->n_ring : 1));
+    memcpy(s->ring, src->ring, sizeof(o_ring) * (size_t)src->n_ring);
+    return s;
+}
+
+o_system *o_subset(const o_system *src, const int64_t *idx, int64_t k) {
+    o_system *s = o_clone(src);
+    for (int64_t q = 0; q < k; ++q) s->m[q] = src->m[idx[q]];
+    s->n = k;
+    return s;
+}
+
+void o_free(o_system *s) {
+    if (!s) return;
+    free(s->m);
+    free(s->gp);
+    free(s->ring);
+    free(s);
+}
+int64_t o_size(const o_system *s) { return s->n; }
+o_mass *o_masses(o_system *s) { return s->m; }
+int o_set_threads(o_system *s, int nthreads) {
+    s->nthreads = nthreads > 0 ? nthreads : 1;
+    return s->nthreads;
+}
