@@ -1,0 +1,107 @@
+"""Benchmark of the force-evaluation + RK12 hot path on one MI355X.
+
+Workload (N=1): the 504-body full-model system (seeded, tests/golden/full504),
+RK12 at dt=300 s.  A step is one RK12 step = 25 force evaluations of the
+system; the timed region is K steps inside one eph_integrate call, with the
+state already resident in HBM.
+
+Prints one JSON line:
+  value   FP64 pair-interactions/s = K * 25 * N*(N-1) / wall time
+  roofline  the dominant kernel class: algorithmic FP64 FLOP per launch /
+            average launch time measured with HIP events on the library's
+            stream over the timed region, against MI355X's FP64 vector peak
+  cpu_baseline  the oracle (C restatement of the reference CPU path, threaded
+            over bodies with OpenMP) on this host's cores, on a bounded sample
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+import numpy as np  # noqa: E402
+
+FP64_VECTOR_PEAK_TFLOPS = 78.6
+DT = 300.0
+# algorithmic FP64 operations (add, sub, mul, div, sqrt, fma = 1 each) per
+# ordered pair, counted from the kernel source (eph_kernels.hip)
+FLOP_MAIN_PAIR = 294          # RELATIVITY 109, tidal (both quadrupoles) 92, Lense-Thirring 83, radiation 10
+FLOP_DEFORM_PAIR = {"nk2": 54, "ndelay": 89, "damped": 217}
+
+
+def load_workload():
+    with np.load(ROOT / "tests" / "golden" / "accel_full504_pool.npz", allow_pickle=False) as z:
+        return z["masses_in"], z["gps"], z["rings"]
+
+
+def cpu_baseline(masses_n, sample_steps):
+    from oracle import pyoracle as po                       # CPU baseline leg only
+    s = po.OSystem.load(ROOT / "tests" / "golden" / "full504" / "Full_Config.txt", pool_at_load=False)
+    cores = min(os.cpu_count() or 1, 16)
+    s.set_threads(cores)
+    t0 = time.perf_counter()
+    s.integrate(DT, sample_steps, pool=True)
+    el = time.perf_counter() - t0
+    pairs = sample_steps * 25 * masses_n * (masses_n - 1)
+    return {"value": pairs / el, "unit": "pair-interactions/s", "cores": cores, "kind": "port",
+            "sample": f"{sample_steps} RK12 steps of the 504-body system ({el:.1f} s)",
+            "rk12_steps_per_s": sample_steps / el}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported (each system, clone or particle "
+              "set fits one MI355X; see DESIGN.md for how the work would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    from ephemeris_amd.system import GpuSystems
+    masses, gps, rings = load_workload()
+    n = len(masses)
+    g = GpuSystems(masses, None, gps, rings)
+    g.integrate(DT, a.warmup)
+    g.profile(True)
+    g.profile_reset()
+    t0 = time.perf_counter()
+    g.integrate(DT, a.steps)
+    el = time.perf_counter() - t0
+    prof = g.profile_read()
+    g.profile(False)
+    pairs = a.steps * 25 * n * (n - 1)
+    value = pairs / el
+
+    tot = {k: v[0] * v[1] for k, v in prof.items()}
+    dominant = max(tot, key=tot.get)
+    main_ms = prof["main_force_pool"][0]
+    flop_main = n * (n - 1) * FLOP_MAIN_PAIR
+    achieved = flop_main / (main_ms * 1e-3) / 1e12
+    roofline = {"kernel": "main_force_pool", "dominant_kernel": dominant, "bound": "fp64_valu",
+                "achieved": achieved, "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / FP64_VECTOR_PEAK_TFLOPS, "traffic": None,
+                "flop_per_launch": flop_main, "avg_launch_ms": main_ms,
+                "kernel_ms": {k: round(v[0], 5) for k, v in prof.items()},
+                "kernel_launches": {k: v[1] for k, v in prof.items()}}
+    out = {"metric": "FP64 pair-interactions/s (504-body RK12)", "value": value,
+           "unit": "pair-interactions/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": el / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64 (double-double state)", "data": "synthetic",
+           "config": {"workload": "full504: 504-body full-model system, RK12 dt=300 s, plain single-system path",
+                      "bodies": n, "parallelism": "single GPU"},
+           "rk12_steps_per_s": a.steps / el, "roofline": roofline}
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(n, a.cpu_steps)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,76 @@
+"""ctypes binding of lib/libephemeris_hip.so (C ABI: include/ephemeris_hip.h).
+
+The HIP library is the only compute path: importing a function of this
+module raises if the library has not been built."""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "lib" / "libephemeris_hip.so"
+
+EPH_OK, EPH_EINVAL, EPH_ENODEV, EPH_ENOMEM, EPH_EDEVICE, EPH_EIO, EPH_ESANITY, EPH_ENOTSUP, EPH_EBUSY = \
+    0, -1, -2, -3, -4, -5, -6, -7, -8
+SUMMATION_POOL, SUMMATION_SERIAL = 0, -1
+
+# every symbol include/ephemeris_hip.h declares: (restype, argtypes)
+P, I, I64, D = C.c_void_p, C.c_int, C.c_int64, C.c_double
+PP = C.POINTER(C.c_void_p)
+SIGNATURES = {
+    "eph_init": (I, [I]),
+    "eph_last_error": (C.c_char_p, []),
+    "eph_abi_version": (I, []),
+    "eph_system_create": (I, [I64, P, P, P, I64, P, I64, P, PP]),
+    "eph_system_destroy": (I, [P]),
+    "eph_system_upload": (I, [P, P, P]),
+    "eph_system_download": (I, [P, P, P]),
+    "eph_system_bodies": (I64, [P]),
+    "eph_system_count": (I64, [P]),
+    "eph_deform": (I, [P]),
+    "eph_accel": (I, [P, I]),
+    "eph_integrate": (I, [P, D, I64]),
+    "eph_integrate_ex": (I, [P, D, I64, I]),
+    "eph_profile_enable": (I, [P, I]),
+    "eph_profile_read": (I, [P, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.POINTER(C.c_int64), I]),
+    "eph_profile_reset": (I, [P]),
+}
+
+
+class EphError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype, fn.argtypes = res, args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc < 0:
+        raise EphError(rc, lib().eph_last_error().decode(errors="replace"))
+    return rc
+
+
+_initialised = False
+
+
+def init(device=0):
+    global _initialised
+    if not _initialised:
+        check(lib().eph_init(device))
+        _initialised = True

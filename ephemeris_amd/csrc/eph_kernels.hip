@@ -1,0 +1,826 @@
+// gfx950 kernels of the force evaluation and the RK12 stage engine.
+//
+// One thread owns one body i and walks its partners j in increasing index
+// order through LDS-staged tiles (every lane reads the same tile entry, a
+// broadcast), so each per-body sum is accumulated in exactly the order of the
+// reference CPU path (src/physics/mass.cpp:112-157, 309-374) and the result
+// is bit-identical to it.  Work is FP64 on the vector ALUs; there is no
+// dense contraction, so no MFMA.
+#include "eph_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+namespace eph {
+
+// ---------------------------------------------------------------------------
+// tables
+// ---------------------------------------------------------------------------
+__constant__ double2 c_rk[325];                    // Feagin RK12, {hi, lo}
+
+__device__ const float g_fz[32 * 64 * 16] = {
+#include "../tables/ring_fzcorr.inc"
+};
+#include "../tables/ring_pade.inc"
+__constant__ double c_xserkk[][2] = EPH_PADE_XSERKK;
+__constant__ double c_xserkm[][2] = EPH_PADE_XSERKM;
+__constant__ double c_xser0[][2] = EPH_PADE_XSER0;
+__constant__ double c_kserkk[][2] = EPH_PADE_KSERKK;
+__constant__ double c_kserkm[][2] = EPH_PADE_KSERKM;
+__constant__ double c_kser0[][2] = EPH_PADE_KSER0;
+
+struct DevSin {
+    __device__ double operator()(double x) const { return ::sin(x); }
+};
+
+// ---------------------------------------------------------------------------
+// field access
+// ---------------------------------------------------------------------------
+struct Acc {
+    double *F;
+    int64_t P;
+    __device__ double ld(int f, int g) const { return F[(int64_t)f * P + g]; }
+    __device__ void st(int f, int g, double v) const { F[(int64_t)f * P + g] = v; }
+    __device__ v3 ldv(int f, int g) const { return v3{ld(f, g), ld(f + 1, g), ld(f + 2, g)}; }
+    __device__ void stv(int f, int g, v3 a) const { st(f, g, a.x); st(f + 1, g, a.y); st(f + 2, g, a.z); }
+    __device__ m3 ldm(int f, int g) const { return m3{ldv(f, g), ldv(f + 3, g), ldv(f + 6, g)}; }
+    __device__ void stm(int f, int g, const m3 &a) const { stv(f, g, a.x); stv(f + 3, g, a.y); stv(f + 6, g, a.z); }
+    __device__ dd ldd(int f, int g) const { return dd{ld(f, g), ld(f + 1, g)}; }
+    __device__ void std_(int f, int g, dd a) const { st(f, g, a.hi); st(f + 1, g, a.lo); }
+    __device__ ddv3 lddv(int f, int g) const { return ddv3{ldd(f, g), ldd(f + 2, g), ldd(f + 4, g)}; }
+    __device__ void stddv(int f, int g, ddv3 a) const { std_(f, g, a.x); std_(f + 2, g, a.y); std_(f + 4, g, a.z); }
+    __device__ ddm3 lddm(int f, int g) const { return ddm3{lddv(f, g), lddv(f + 6, g), lddv(f + 12, g)}; }
+    __device__ void stddm(int f, int g, const ddm3 &a) const { stddv(f, g, a.x); stddv(f + 6, g, a.y); stddv(f + 12, g, a.z); }
+};
+
+// ---------------------------------------------------------------------------
+// geopotential::sum (src/physics/geopotential.impl:64-144)
+// ---------------------------------------------------------------------------
+__device__ v3 gp_sum(const eph_geopotential &gp, double R, v3 r) {
+    double rho2 = r.x * r.x + r.y * r.y;
+    double rr2 = 1 / (rho2 + r.z * r.z);
+    double rho1 = sqrt(rho2);
+    double rr1 = sqrt(rr2);
+    double rb1 = R * rr1;
+    double rbn = rb1;
+    double cphi = rho1 * rr1;
+    double sphi = r.z * rr1;
+    double cm, sm;
+    if (rho2 < 2.2250738585072014e-308) { cm = 1; sm = 0; }
+    else { cm = r.x / rho1; sm = r.y / rho1; }
+    double pnm[EPH_GP_MAX_DEGREE], pn1m[EPH_GP_MAX_DEGREE];
+    pnm[0] = 0;
+    pn1m[0] = 1;
+    double ppn = 1, pn = 1, pn1 = sphi;
+    double zx = 0, zz = 0;
+    v3 t{0, 0, 0};
+    const int Nz = (int)gp.Nz, Nt = (int)gp.Nt;
+    for (int n = 2; n <= Nz; ++n) {
+        rbn *= rb1;
+        const double *JCSn = gp.c + (n * n - 4);
+        pn = ((double)(2 * n - 1) * sphi * pn1 - (double)(n - 1) * pn) / (double)n;
+        ppn = sphi * ppn + (double)n * pn1;
+        double z1 = JCSn[0] * rbn;
+        zx += (double)(n + 1) * z1 * pn;
+        zz -= z1 * ppn;
+        { double sw = pn; pn = pn1; pn1 = sw; }
+        if (n > Nt) continue;
+        v3 f{0, 0, 0};
+        double m1 = (double)(2 * n - 1) * sphi;
+        double m2 = (double)(-n) * sphi;
+        double smm = 0, cmm = 0, smm1 = 0, cmm1 = 0;
+        for (int m = 1; m <= n; ++m) {
+            if (m == n) {
+                pn1m[m - 1] = 0;
+                pnm[m - 1] = (double)(2 * n - 1) * cphi * pnm[m - 2];
+            } else
+                pnm[m - 1] = (m1 * pn1m[m - 1] - (double)(n + m - 1) * pnm[m - 1]) / (double)(n - m);
+            if (m == 1) { smm = sm; cmm = cm; smm1 = 0; cmm1 = 1; }
+            else { smm = 2 * cm * smm1 - smm; cmm = 2 * cm * cmm1 - cmm; }
+            double t1 = JCSn[2 * m] * smm + JCSn[2 * m - 1] * cmm;
+            double t2 = JCSn[2 * m] * cmm - JCSn[2 * m - 1] * smm;
+            double ppnm = m2 * pnm[m - 1] + (double)(n + m) * pn1m[m - 1];
+            f.x += t1 * pnm[m - 1];
+            f.y += (double)m * t2 * pnm[m - 1];
+            f.z += t1 * ppnm;
+            { double sw = cmm; cmm = cmm1; cmm1 = sw; }
+            { double sw = smm; smm = smm1; smm1 = sw; }
+            { double sw = pnm[m - 1]; pnm[m - 1] = pn1m[m - 1]; pn1m[m - 1] = sw; }
+        }
+        f.x *= (double)(n + 1);
+        t += rbn * f;
+    }
+    t.x = zx - cphi * t.x;
+    t.z += zz * cphi;
+    t = rr2 * t;
+    double g1 = t.x * cphi - t.z * sphi;
+    return v3{g1 * cm - t.y * sm, g1 * sm + t.y * cm, cphi * t.z + sphi * t.x};
+}
+
+// ---------------------------------------------------------------------------
+// ring::sum (src/physics/ring.impl:2070-2196)
+// ---------------------------------------------------------------------------
+__device__ float fzcorr(float x, float y, int kh) {
+    const int N = 32;
+    x *= (float)N;
+    y *= (float)N;
+    int i = (int)floorf(x), j = (int)floorf(y);
+    j -= kh * N;
+    if (i < 0) i = 0;
+    if (j < 0) j = 0;
+    if (i >= N) i = N - 1;
+    if (j >= N) j = N - 1;
+    j += kh * N;
+    x -= (float)i;
+    y -= (float)j;
+    const float *p = g_fz + 16 * (2 * N * i + j);
+    return p[0] + x * (p[4] + x * (p[8] + x * p[12]))
+         + y * (p[1] + x * (p[5] + x * (p[9] + x * p[13]))
+         + y * (p[2] + x * (p[6] + x * (p[10] + x * p[14]))
+         + y * (p[3] + x * (p[7] + x * (p[11] + x * p[15])))));
+}
+template <int NC>
+__device__ double padesum(double x, const double (&coef)[NC][2]) {
+    double resn = 0, resd = 0;
+    int n = NC;
+    while (n > 0) {
+        --n;
+        resn = x * resn + coef[n][0];
+        resd = x * resd + coef[n][1];
+    }
+    return resn / resd;
+}
+
+__device__ v3 ring_sum(const eph_ring &rg, double host_R, v3 r) {
+    const double pi = kPi;
+    bool psign = r.z < 0;
+    if (psign) r.z = -r.z;
+    const double re2 = r.x * r.x + r.y * r.y;
+    const double re = sqrt(re2);
+    const double rz2 = r.z * r.z;
+    const double r2 = re2 + rz2;
+    const double rz = r.z;
+    v3 ret{0, 0, 0};
+    const int N = (int)rg.N;
+    for (int i = 0; i < N; ++i) {
+        double R = rg.R_R[i] * host_R;
+        double H = rg.H_R[i] * host_R;
+        double reR = re * R;
+        double R2 = R * R, H2, fzh = 1;
+        double k2dre = 4 / (r2 + R2 + 2 * reR);
+        if (H > 0) {
+            H2 = H * H;
+            k2dre -= H2 * (k2dre * k2dre) * (1 - k2dre * rz2) / 48;
+            fzh = (re > R ? re - R : 0);
+            fzh = 4 * (rz2 + fzh * fzh);
+            fzh = sqrt(fzh / (fzh + H2));
+        }
+        double k2 = reR * k2dre;
+        k2dre *= R2 * sqrt(k2dre);
+        double phi = atan2(rz, re - R);
+        double k = 0, lk = 0, fx;
+        bool lkh = k2 >= 0.45;
+        if (lkh) {
+            k = sqrt(k2);
+            lk = log((1 - k2) / (8 * (1 + k)));
+            fx = (padesum(k, c_xserkk) + lk * padesum(k, c_xserkm)) / (k2 * k2);
+        } else {
+            fx = padesum(k2, c_xser0);
+        }
+        fx *= k2dre;
+        double fz;
+        double s = sin(phi), c = cos(phi);
+        bool kh = k2 >= .5;
+        bool exphi;
+        if (kh) {
+            const double phimax = (pi / 2);
+            exphi = c > 0;
+            double ck = 1 - k2, lk2 = log(ck / 16), sk = sqrt(ck);
+            double cc = exphi ? -c : c;
+            double cphi = exphi ? pi - phi : phi;
+            fz = cphi - (s / 8) * ((lk2 * (k2 - 5) - 2 * ck) * sk + 2 * cc * (1 + lk2) * ck);
+            fz *= fzcorr((float)((exphi ? phi : pi - phi) * (1 / phimax)), (float)(k2 * 2), 1);
+        } else {
+            const double phimax = (809. / 512);
+            exphi = phi > phimax;
+            double cr2 = exphi ? R2 + rz2 : r2;
+            double cRdr2 = (exphi ? re2 : R2) / cr2;
+            fz = (pi / 8) * (4 - 3 * cRdr2) * (rz / sqrt(cr2) * cRdr2);
+            fz *= fzcorr((float)((exphi ? pi - phi : phi) * (1 / phimax)), (float)(k2 * 2), 0);
+        }
+        if (exphi) {
+            double ek;
+            if (lkh) ek = padesum(k, c_kserkk) + lk * padesum(k, c_kserkm);
+            else ek = padesum(k2, c_kser0);
+            fz = pi - 2 * sqrt(1 - k2) * ek * s - fz;
+        }
+        double Gs = rg.GsR2[i] / (host_R * host_R);
+        fx *= Gs;
+        fz *= Gs;
+        ret += v3{-4 * fx * r.x, -4 * fx * r.y, -2 * fz * fzh};
+    }
+    if (psign) ret.z = -ret.z;
+    return ret;
+}
+
+// world-frame acceleration (per unit GM of the non-point body p) at offset r
+// from p: geopotential (mass.cpp:186-188) or ring (mass.cpp:200-203)
+__device__ v3 nonpoint_field(const View &d, const Acc &A, int p, int kind, int model, v3 r) {
+    double R = A.ld(F_RAD, p);
+    if (kind == 0) {
+        m3 fmis = todbl(A.lddm(F_S, p));
+        v3 lr = tolocal(fmis, r);
+        return fmis * gp_sum(d.gps[model], R, lr);
+    }
+    v3 migl = todbl(A.lddv(F_GL, p));
+    v3 x = asc_node(migl), z = unit(migl);
+    m3 fgls{x, cross(z, x), z};
+    v3 lr = tolocal(fgls, r);
+    return fgls * ring_sum(d.rings[model], R, lr);
+}
+
+
+// FINALIZE_RELATIVITY and RING_CORRECTION (mass.impl:133-152) of body g with
+// its final daccel/dtorque; writes phi, naccel, daccel, dtorque
+__device__ void finalize_body(const View &d, const Acc &A, int g, v3 da, v3 dq) {
+    A.st(F_PHI, g, A.ld(F_PHIC, g) * kC2);
+    const v3 nacc = A.ldv(F_NACCC, g) * kC2;
+    A.stv(F_NACC, g, nacc);
+    const int rg = d.np_ring[g];
+    if (rg >= 0) {
+        const eph_ring &mr = d.rings[rg];
+        da = da - mr.GM_ratio * ((A.ldv(F_GACC, g) + da) + nacc);
+        v3 mGL = todbl(A.lddv(F_GL, g));
+        const double rmGL = normalize(mGL);
+        const v3 ptorque = dot(dq, mGL) * mGL;
+        v3 dtq = dq - ptorque;
+        dtq = (rmGL / (rmGL + mr.GL_R2 * A.ld(F_R2, g))) * dtq;
+        dq = dtq + ptorque;
+    }
+    A.stv(F_DACC, g, da);
+    A.stv(F_DTQ, g, dq);
+}
+
+// ---------------------------------------------------------------------------
+// deform + spin fixed point + PREPARE_RELATIVITY (mass.cpp:78-135)
+// ---------------------------------------------------------------------------
+struct JDeform { double rxh, rxl, ryh, ryl, rzh, rzl, vxh, vxl, vyh, vyl, vzh, vzl, GM, sR2, tdf, pad; };
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_deform(View d, const int2 *__restrict__ blk, int prepare) {
+    const Acc A{d.F, d.nbp};
+    const int s = blk[blockIdx.x].x, i0 = blk[blockIdx.x].y;
+    const SysMeta sm = d.sys[s];
+    const int n = sm.n, off = sm.off;
+    const int il = i0 + (int)threadIdx.x;
+    const bool valid = il < n;
+    const int gi = off + (valid ? il : 0);
+    __shared__ JDeform tile[BS];
+
+    const ddv3 ri = A.lddv(F_R, gi), vi = A.lddv(F_V, gi);
+    v3 w = A.ldv(F_W, gi);
+    const double sR2 = A.ld(F_SR2, gi), k2 = A.ld(F_K2, gi), td = A.ld(F_TD, gi);
+    const int cls = k2 != 0 ? (td != 0 ? 0 : 1) : 2;
+    double phi = 0;
+    v3 nacc{0, 0, 0};
+    m3 C = mzero();
+    int iters_left = 4;                                    // MAX_ANGULAR_VELOCITY_ITER
+    bool active = valid;
+    for (;;) {
+        if (active) {
+            phi = 0;
+            nacc = v3{0, 0, 0};
+            C = mzero();
+        }
+        for (int t0 = 0; t0 < n; t0 += BS) {
+            __syncthreads();
+            {
+                const int jl = t0 + (int)threadIdx.x;
+                if (jl < n) {
+                    const int gj = off + jl;
+                    JDeform &T = tile[threadIdx.x];
+                    T.rxh = A.ld(F_R + 0, gj); T.rxl = A.ld(F_R + 1, gj);
+                    T.ryh = A.ld(F_R + 2, gj); T.ryl = A.ld(F_R + 3, gj);
+                    T.rzh = A.ld(F_R + 4, gj); T.rzl = A.ld(F_R + 5, gj);
+                    T.vxh = A.ld(F_V + 0, gj); T.vxl = A.ld(F_V + 1, gj);
+                    T.vyh = A.ld(F_V + 2, gj); T.vyl = A.ld(F_V + 3, gj);
+                    T.vzh = A.ld(F_V + 4, gj); T.vzl = A.ld(F_V + 5, gj);
+                    T.GM = A.ld(F_GM, gj); T.sR2 = A.ld(F_SR2, gj); T.tdf = A.ld(F_TDF, gj);
+                }
+            }
+            __syncthreads();
+            if (!active) continue;
+            const int tn = min(BS, n - t0);
+            for (int q = 0; q < tn; ++q) {
+                const int jl = t0 + q;
+                if (jl == il) continue;
+                const JDeform &T = tile[q];
+                const ddv3 rj{dd{T.rxh, T.rxl}, dd{T.ryh, T.ryl}, dd{T.rzh, T.rzl}};
+                const v3 r = diff(rj, ri);
+                const double rr2 = 1 / smax(dot(r, r), sR2 + T.sR2);
+                const double rr = sqrt(rr2);
+                const double tp_dphi = rr * T.GM;
+                if (cls == 2) {                            // _NK2, mass.impl:38-46
+                    const double tp_dg = rr2 * tp_dphi;
+                    phi -= tp_dphi;
+                    nacc += tp_dg * r;
+                } else if (cls == 1) {                     // _NDELAY, mass.impl:25-36
+                    phi -= tp_dphi;
+                    const double tp_dg = rr2 * tp_dphi;
+                    nacc += tp_dg * r;
+                    const double fmj = -3 * tp_dg;
+                    C = C - (fmj * k2) * subdiag(outer(rr2 * r, r), 1.0 / 3);
+                } else {                                   // damped, mass.impl:1-23
+                    const ddv3 vj{dd{T.vxh, T.vxl}, dd{T.vyh, T.vyl}, dd{T.vzh, T.vzl}};
+                    const v3 v = diff(vj, vi);
+                    phi -= tp_dphi;
+                    const double tp_dg = rr2 * tp_dphi;
+                    nacc += tp_dg * r;
+                    const double fmj = -3 * tp_dg;
+                    v3 dw = rr2 * cross(r, cross(w, r) - v);
+                    double dwt = normalize(dw);
+                    dwt *= td * T.tdf;
+                    const double ecwt2 = 1 + 4 * dwt * dwt,
+                                 secwt2 = sqrt(ecwt2),
+                                 x0 = sqrt((1 + secwt2) / (2 * ecwt2)),
+                                 y0 = dwt * sqrt(2 / (ecwt2 * (1 + secwt2))),
+                                 z02 = dwt * dwt * (2 / (ecwt2 + secwt2));
+                    const v3 dr = x0 * r + y0 * cross(dw, r);
+                    m3 t = outer(rr2 * dr, dr) - outer(z02 * dw, dw);
+                    t = adddiag(t, z02 - 1.0 / 3);
+                    C = C - (fmj * k2) * t;
+                }
+            }
+        }
+        bool again = false;
+        if (active) {
+            // UPDATE_HARMONICS, mass.impl:48-61
+            const double k2r = A.ld(F_K2R, gi), Rr = A.ld(F_RAD, gi), R2 = A.ld(F_R2, gi);
+            const double GM = A.ld(F_GM, gi), exJ2 = A.ld(F_EXJ2, gi), Ai = A.ld(F_A, gi);
+            const double w2 = dot(w, w);
+            C = adddiag(C, w2 * k2r / 3);
+            C = C - outer(k2r * w, w);
+            C = (Rr * R2 / (2 * GM)) * C;
+            const m3 fmis = todbl(A.lddm(F_S, gi));
+            m3 mc = A.ldm(F_CST, gi);
+            mc.x.x += exJ2 / 2;
+            mc.y.y += exJ2 / 2;
+            mc.z.z -= exJ2;
+            C = C + toworld(fmis, mc);
+            const m3 GI = (-2.0 / 3 * R2) * subdiag(C, Ai);
+            // UPDATE_ANGULAR_VELOCITY, mass.impl:63-72 (not part of msystem::deform)
+            const ddv3 GL = A.lddv(F_GL, gi);
+            bool brk;
+            if (!prepare) {
+                brk = true;
+            } else if (finite(GL)) {
+                const v3 oldw = w;
+                w = inverse(GI) * todbl(GL);
+                const v3 dwv = w - oldw;
+                brk = dot(dwv, dwv) < 1e-18 * dot(oldw, oldw);
+            } else {
+                A.stddv(F_GL, gi, mkddv(GI * w));
+                brk = true;
+            }
+            A.stm(F_GI, gi, GI);
+            again = !brk && --iters_left > 0;
+            if (!again) active = false;
+        }
+        if (!__syncthreads_or(again)) break;
+    }
+    if (!valid) return;
+    A.stm(F_CPOT, gi, C);
+    A.stv(F_W, gi, w);
+    if (prepare) {                                         // PREPARE_RELATIVITY, mass.impl:76-80
+        const v3 beta = todbl(vi) / kC;
+        A.stv(F_BETA, gi, beta);
+        A.st(F_BETA2, gi, dot(beta, beta));
+        A.st(F_PHIC, gi, phi / kC2);
+        A.stv(F_NACCC, gi, nacc / kC2);
+    } else {                                               // plain msystem::deform()
+        A.st(F_PHI, gi, phi);
+        A.stv(F_NACC, gi, nacc);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// main force, thread-pool summation order (mass.cpp:136-157)
+// ---------------------------------------------------------------------------
+struct JMain {
+    double rxh, rxl, ryh, ryl, rzh, rzl, GM, sR2;
+    double phi, beta2, bx, by, bz, nx, ny, nz;
+    double C[9], R2, GLx, GLy, GLz, lum, pad0, pad1;
+};
+
+__device__ __forceinline__ void load_jmain(const Acc &A, JMain &T, int gj) {
+    T.rxh = A.ld(F_R + 0, gj); T.rxl = A.ld(F_R + 1, gj);
+    T.ryh = A.ld(F_R + 2, gj); T.ryl = A.ld(F_R + 3, gj);
+    T.rzh = A.ld(F_R + 4, gj); T.rzl = A.ld(F_R + 5, gj);
+    T.GM = A.ld(F_GM, gj); T.sR2 = A.ld(F_SR2, gj);
+    T.phi = A.ld(F_PHIC, gj); T.beta2 = A.ld(F_BETA2, gj);
+    T.bx = A.ld(F_BETA, gj); T.by = A.ld(F_BETA + 1, gj); T.bz = A.ld(F_BETA + 2, gj);
+    T.nx = A.ld(F_NACCC, gj); T.ny = A.ld(F_NACCC + 1, gj); T.nz = A.ld(F_NACCC + 2, gj);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) T.C[q] = A.ld(F_CPOT + q, gj);
+    T.R2 = A.ld(F_R2, gj);
+    T.GLx = todbl(dd{A.ld(F_GL + 0, gj), A.ld(F_GL + 1, gj)});
+    T.GLy = todbl(dd{A.ld(F_GL + 2, gj), A.ld(F_GL + 3, gj)});
+    T.GLz = todbl(dd{A.ld(F_GL + 4, gj), A.ld(F_GL + 5, gj)});
+    T.lum = A.ld(F_LUM, gj);
+}
+
+// own-body data of the main force
+struct IMain {
+    ddv3 r;
+    double sR2, phi, beta2, R2, GM, rR2G4c;
+    v3 beta, GL;
+    m3 C;
+};
+__device__ __forceinline__ IMain load_imain(const Acc &A, int gi) {
+    IMain I;
+    I.r = A.lddv(F_R, gi);
+    I.sR2 = A.ld(F_SR2, gi); I.phi = A.ld(F_PHIC, gi); I.beta2 = A.ld(F_BETA2, gi);
+    I.R2 = A.ld(F_R2, gi); I.GM = A.ld(F_GM, gi); I.rR2G4c = A.ld(F_RR2G4C, gi);
+    I.beta = A.ldv(F_BETA, gi);
+    I.GL = todbl(A.lddv(F_GL, gi));
+    I.C = A.ldm(F_CPOT, gi);
+    return I;
+}
+
+struct PairGeo { v3 r, b; double rr2, rr, rr3, tp_dg; };
+
+// RELATIVITY (mass.impl:82-97)
+__device__ __forceinline__ PairGeo relativity(const IMain &I, const JMain &T, v3 &gaccel) {
+    PairGeo p;
+    const ddv3 rj{dd{T.rxh, T.rxl}, dd{T.ryh, T.ryl}, dd{T.rzh, T.rzl}};
+    p.r = diff(rj, I.r);
+    p.rr2 = 1 / smax(dot(p.r, p.r), I.sR2 + T.sR2);
+    p.rr = sqrt(p.rr2);
+    p.rr3 = p.rr * p.rr2;
+    const double tp_dphi = p.rr * T.GM;
+    p.tp_dg = p.rr2 * tp_dphi;
+    const v3 bj{T.bx, T.by, T.bz}, nj{T.nx, T.ny, T.nz};
+    const double rbj = dot(p.r, bj);
+    double tp_rbjrr2 = rbj * p.rr;
+    tp_rbjrr2 *= tp_rbjrr2;
+    const double delta1 = 4 * I.phi + T.phi + I.beta2 + 2 * T.beta2
+                        - 4 * dot(I.beta, bj) + (dot(p.r, nj) - 3 * tp_rbjrr2) / 2;
+    p.b = bj - I.beta;
+    gaccel += ((7 * tp_dphi / 2) * nj + (p.tp_dg * delta1) * p.r)
+              + (p.tp_dg * (rbj - dot(p.r, p.b) * 4)) * p.b;
+    return p;
+}
+
+__device__ __forceinline__ v3 quad_field(const m3 &C, const PairGeo &p) {
+    const v3 Cr = C * p.r;
+    return (dot(p.r, Cr) * 5 * p.rr2) * p.r - (Cr + Cr);
+}
+
+// LENSE_THIRRING (mass.impl:99-106) and RADIATION_PRESSURE (:108-109)
+__device__ __forceinline__ void lt_rad(const IMain &I, const JMain &T, const PairGeo &p, v3 &da, v3 &dq) {
+    const double rcr32 = 2 * T.GM / kC * p.rr3;
+    const v3 GLb = cross(I.GL, p.b);
+    da -= rcr32 * (GLb - (3 * dot(GLb, p.r) * p.rr2) * p.r);
+    dq -= rcr32 * cross(I.GL, cross(p.r, p.b));
+    const v3 GLj{T.GLx, T.GLy, T.GLz};
+    da += cross(rcr32 * (GLj - (3 * dot(GLj, p.r) * p.rr2) * p.r), p.b);
+    da += (T.lum * I.rR2G4c * p.rr3 / I.GM) * p.r;
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_main_pool(View d, const int2 *__restrict__ blk) {
+    const Acc A{d.F, d.nbp};
+    const int s = blk[blockIdx.x].x, i0 = blk[blockIdx.x].y;
+    const SysMeta sm = d.sys[s];
+    const int n = sm.n, off = sm.off;
+    const int il = i0 + (int)threadIdx.x;
+    const bool valid = il < n;
+    const int gi = off + (valid ? il : 0);
+    __shared__ JMain tile[BS];
+    const IMain I = load_imain(A, gi);
+    v3 ga{0, 0, 0}, da{0, 0, 0}, dq{0, 0, 0};
+    double mind = A.ld(F_MIND, gi), maxi = A.ld(F_MAXI, gi);
+    for (int t0 = 0; t0 < n; t0 += BS) {
+        __syncthreads();
+        if (t0 + (int)threadIdx.x < n) load_jmain(A, tile[threadIdx.x], off + t0 + (int)threadIdx.x);
+        __syncthreads();
+        if (!valid) continue;
+        const int tn = min(BS, n - t0);
+        for (int q = 0; q < tn; ++q) {
+            if (t0 + q == il) continue;
+            const JMain &T = tile[q];
+            const PairGeo p = relativity(I, T, ga);
+            mind = checked_max_into(mind, p.rr);
+            maxi = checked_max_into(maxi, p.tp_dg);
+            // ROTATIONAL_TIDAL_DEFORMATION_NANTI_FORCE, mass.impl:118-125
+            const v3 dg = (T.GM * p.rr3 * p.rr2 * I.R2) * quad_field(I.C, p);
+            da += dg;
+            dq += cross(p.r, dg);
+            const m3 Cj{v3{T.C[0], T.C[1], T.C[2]}, v3{T.C[3], T.C[4], T.C[5]}, v3{T.C[6], T.C[7], T.C[8]}};
+            da += (T.GM * p.rr3 * p.rr2 * T.R2) * quad_field(Cj, p);
+            lt_rad(I, T, p, da, dq);
+        }
+    }
+    if (!valid) return;
+    A.stv(F_GACC, gi, ga);
+    A.stv(F_DACC, gi, da);
+    A.stv(F_DTQ, gi, dq);
+    A.st(F_MIND, gi, mind);
+    A.st(F_MAXI, gi, maxi);
+    if (sm.nitem == 0) {                                   // FINALIZE_RELATIVITY
+        A.st(F_PHI, gi, I.phi * kC2);
+        A.stv(F_NACC, gi, A.ldv(F_NACCC, gi) * kC2);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// main force, single-thread summation order (mass.cpp:329-374): the pair
+// terms of partner p acting on body i that the reference adds as reactions
+// while visiting p (ROTATIONAL_TIDAL_DEFORMATION and APPLY_NONPOINT_FORCE)
+// are added before i's own partner loop when p < i and after it when p > i.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void serial_reaction(const View &d, const Acc &A, int gp_, int gi, const JMain &T,
+                                                const IMain &I, v3 &da) {
+    // outer body p visiting partner i: r = r_i - r_p
+    const ddv3 rp{dd{T.rxh, T.rxl}, dd{T.ryh, T.ryl}, dd{T.rzh, T.rzl}};
+    PairGeo p;
+    p.r = diff(I.r, rp);
+    p.rr2 = 1 / smax(dot(p.r, p.r), T.sR2 + I.sR2);
+    p.rr = sqrt(p.rr2);
+    p.rr3 = p.rr * p.rr2;
+    const m3 Cp{v3{T.C[0], T.C[1], T.C[2]}, v3{T.C[3], T.C[4], T.C[5]}, v3{T.C[6], T.C[7], T.C[8]}};
+    const v3 dg = (p.rr3 * p.rr2 * T.R2) * quad_field(Cp, p);
+    da -= T.GM * dg;
+    const int gpi = d.np_gp[gp_], rgi = d.np_ring[gp_];
+    if (gpi >= 0) da += T.GM * nonpoint_field(d, A, gp_, 0, gpi, p.r);
+    if (rgi >= 0) da += T.GM * nonpoint_field(d, A, gp_, 1, rgi, p.r);
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_main_serial(View d, const int2 *__restrict__ blk) {
+    const Acc A{d.F, d.nbp};
+    const int s = blk[blockIdx.x].x, i0 = blk[blockIdx.x].y;
+    const SysMeta sm = d.sys[s];
+    const int n = sm.n, off = sm.off;
+    const int il = i0 + (int)threadIdx.x;
+    const bool valid = il < n;
+    const int gi = off + (valid ? il : 0);
+    __shared__ JMain tile[BS];
+    const IMain I = load_imain(A, gi);
+    const int my_gp = d.np_gp[gi], my_ring = d.np_ring[gi];
+    v3 ga{0, 0, 0}, da{0, 0, 0}, dq{0, 0, 0};
+    double mind = A.ld(F_MIND, gi), maxi = A.ld(F_MAXI, gi);
+    // phase 0: reactions from partners before i; 1: own loop; 2: reactions after i
+    for (int phase = 0; phase < 3; ++phase) {
+        for (int t0 = 0; t0 < n; t0 += BS) {
+            __syncthreads();
+            if (t0 + (int)threadIdx.x < n) load_jmain(A, tile[threadIdx.x], off + t0 + (int)threadIdx.x);
+            __syncthreads();
+            if (!valid) continue;
+            const int tn = min(BS, n - t0);
+            for (int q = 0; q < tn; ++q) {
+                const int jl = t0 + q;
+                if (jl == il) continue;
+                const JMain &T = tile[q];
+                if (phase == 1) {
+                    const PairGeo p = relativity(I, T, ga);
+                    mind = checked_max_into(mind, p.rr);
+                    maxi = checked_max_into(maxi, p.tp_dg);
+                    // ROTATIONAL_TIDAL_DEFORMATION, mass.impl:111-116
+                    const v3 dg = (p.rr3 * p.rr2 * I.R2) * quad_field(I.C, p);
+                    da += T.GM * dg;
+                    dq += T.GM * cross(p.r, dg);
+                    lt_rad(I, T, p, da, dq);
+                    if (my_gp >= 0) {                      // APPLY_NONPOINT_FORCE, mass.impl:127-130
+                        const v3 an = nonpoint_field(d, A, gi, 0, my_gp, p.r);
+                        da -= T.GM * an;
+                        dq -= T.GM * cross(p.r, an);
+                    }
+                    if (my_ring >= 0) {
+                        const v3 an = nonpoint_field(d, A, gi, 1, my_ring, p.r);
+                        da -= T.GM * an;
+                        dq -= T.GM * cross(p.r, an);
+                    }
+                } else if ((phase == 0) == (jl < il)) {
+                    serial_reaction(d, A, off + jl, gi, T, I, da);
+                }
+            }
+        }
+    }
+    if (!valid) return;
+    A.stv(F_GACC, gi, ga);
+    A.st(F_MIND, gi, mind);
+    A.st(F_MAXI, gi, maxi);
+    // FINALIZE_RELATIVITY and RING_CORRECTION (mass.impl:133-152)
+    finalize_body(d, A, gi, da, dq);
+}
+
+// ---------------------------------------------------------------------------
+// non-point items, thread-pool order (mass.cpp:158-214, 253-307)
+// ---------------------------------------------------------------------------
+__global__ void k_np_field(View d, int nrows) {
+    // one thread per (item, partner): an = field of the item at r_j - r_p
+    const Acc A{d.F, d.nbp};
+    const int row = blockIdx.y;
+    if (row >= nrows) return;
+    const NPItem it = d.items[row];
+    const SysMeta sm = d.sys[it.sys];
+    const int jl = blockIdx.x * blockDim.x + threadIdx.x;
+    if (jl >= sm.n) return;
+    const int gj = sm.off + jl;
+    v3 an{0, 0, 0};
+    if (gj != it.body) {
+        const v3 r = diff(A.lddv(F_R, gj), A.lddv(F_R, it.body));
+        an = nonpoint_field(d, A, it.body, it.kind, it.model, r);
+    }
+    double *o = d.an + 3 * ((int64_t)it.an_off + jl);
+    o[0] = an.x; o[1] = an.y; o[2] = an.z;
+}
+
+__global__ void k_np_item(View d, int nrows, int maxT) {
+    // one thread per (item, round k): the item's partial sums over the slice
+    // it visits in round k (tpmi of accel_nonpoint)
+    const Acc A{d.F, d.nbp};
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = row % maxT, it_i = row / maxT;
+    if (it_i >= nrows) return;
+    const NPItem it = d.items[it_i];
+    const SysMeta sm = d.sys[it.sys];
+    const int T = sm.T;
+    if (k >= T) return;
+    int kk = it.chain + k;
+    if (kk >= T) kk -= T;
+    const int j0 = kk * sm.n / T, j1 = (kk + 1) * sm.n / T;
+    const ddv3 rp = A.lddv(F_R, it.body);
+    v3 td{0, 0, 0}, tq{0, 0, 0};
+    for (int jl = j0; jl < j1; ++jl) {
+        const int gj = sm.off + jl;
+        if (gj == it.body) continue;
+        const v3 r = diff(A.lddv(F_R, gj), rp);
+        const double *a = d.an + 3 * ((int64_t)it.an_off + jl);
+        const v3 an{a[0], a[1], a[2]};
+        const double GMj = A.ld(F_GM, gj);
+        td -= GMj * an;
+        tq -= GMj * cross(r, an);
+    }
+    double *o = d.part + 6 * ((int64_t)it_i * maxT + k);
+    o[0] = td.x; o[1] = td.y; o[2] = td.z; o[3] = tq.x; o[4] = tq.y; o[5] = tq.z;
+}
+
+__global__ void k_np_recv(View d, const int2 *__restrict__ blk, int maxT) {
+    // one thread per body of a system with items: receiver terms in round
+    // order, then the body's own item sums, finalisation and ring correction
+    const Acc A{d.F, d.nbp};
+    const int s = blk[blockIdx.x].x, i0 = blk[blockIdx.x].y;
+    const SysMeta sm = d.sys[s];
+    const int jl = i0 + (int)threadIdx.x;
+    if (jl >= sm.n) return;
+    const int gj = sm.off + jl;
+    const int T = sm.T, nit = sm.nitem, n = sm.n;
+    v3 da = A.ldv(F_DACC, gj), dq = A.ldv(F_DTQ, gj);
+    int sj = 0;
+    while ((sj + 1) * n / T <= jl) ++sj;
+    for (int k = 0; k < T; ++k) {
+        int c = sj - k;
+        if (c < 0) c += T;
+        const int first = c * nit / T, last = (c + 1) * nit / T;
+        for (int q = last - 1; q >= first; --q) {
+            const NPItem it = d.items[sm.item0 + q];
+            if (it.body == gj) continue;
+            const double *a = d.an + 3 * ((int64_t)it.an_off + jl);
+            da += A.ld(F_GM, it.body) * v3{a[0], a[1], a[2]};
+        }
+    }
+    // own items: idaccel (geopotential) then jdaccel (ring), mass.cpp:298-307
+    for (int kind = 0; kind < 2; ++kind) {
+        for (int q = 0; q < nit; ++q) {
+            const NPItem it = d.items[sm.item0 + q];
+            if (it.body != gj || it.kind != kind) continue;
+            v3 sa{0, 0, 0}, sq{0, 0, 0};
+            for (int k = 0; k < T; ++k) {
+                const double *p = d.part + 6 * ((int64_t)(sm.item0 + q) * maxT + k);
+                sa += v3{p[0], p[1], p[2]};
+                sq += v3{p[3], p[4], p[5]};
+            }
+            A.stv(kind == 0 ? F_IDACC : F_JDACC, gj, sa);
+            A.stv(kind == 0 ? F_IDTQ : F_JDTQ, gj, sq);
+            da += sa;
+            dq += sq;
+        }
+    }
+    finalize_body(d, A, gj, da, dq);
+}
+
+// ---------------------------------------------------------------------------
+// RK12 stage: store f[k-1], combine, update state (RungeKutta.impl:113-186)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_stage(View d, int k, double dt, double dt_k, double tadd) {
+    const Acc A{d.F, d.nbp};
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= d.nb) return;
+    ddv3 v = A.lddv(F_V, g);
+    if (k == 1) {
+        A.stddv(F_X0R, g, A.lddv(F_R, g));
+        A.stddv(F_X0V, g, v);
+        A.stddv(F_X0GL, g, A.lddv(F_GL, g));
+        A.stddm(F_X0S, g, A.lddm(F_S, g));
+        A.stv(F_X0W, g, A.ldv(F_W, g));
+        A.stv(F_EGRAD, g, A.ldv(F_DTQ, g));
+    }
+    const v3 w_now = A.ldv(F_W, g), tq_now = A.ldv(F_DTQ, g);
+    {
+        const int fb = F_STAGE + (k - 1) * kStageFields;
+        A.stddv(fb, g, v);
+        A.stv(fb + 6, g, (A.ldv(F_GACC, g) + A.ldv(F_DACC, g)) + A.ldv(F_NACC, g));
+        A.stv(fb + 9, g, w_now);
+        A.stv(fb + 12, g, tq_now);
+    }
+    v3 egrad = A.ldv(F_EGRAD, g);
+    if (k == 25) {
+        const m3 GI = A.ldm(F_GI, g);
+        const m3 ir = inverse(GI);
+        const double irdet = det(ir);
+        const double dt2 = dt * dt;
+        const v3 tq = tq_now * dt2;
+        const v3 tqdot = dt2 * (tq_now - egrad);
+        v3 j = todbl(A.lddv(F_GL, g));
+        const v3 w = w_now * dt;
+        const v3 irtq = ir * tq;
+        j = j * dt;
+        const v3 t1 = dot(w, j) * (ir * w + (GI * j) * irdet);
+        egrad = cross((t1 + cross(w, irtq)) + ir * ((cross(j, irtq) - 2 * cross(w, tq)) + tqdot), w);
+        A.stv(F_EGRAD, g, egrad);
+    }
+    ddv3 cv = mkddv(v3{0, 0, 0});
+    v3 ca{0, 0, 0}, cw{0, 0, 0}, ct{0, 0, 0};
+    const int base = k * (k - 1) / 2;
+    for (int j = 0; j < k; ++j) {
+        const double2 c = c_rk[base + j];
+        if (c.x == 0) continue;
+        const dd ckj{c.x, c.y};
+        const double fc = c.x + c.y;
+        const int fb = F_STAGE + j * kStageFields;
+        cv = add(cv, scale(ckj, A.lddv(fb, g)));
+        ca += fc * A.ldv(fb + 6, g);
+        cw += fc * A.ldv(fb + 9, g);
+        ct += fc * A.ldv(fb + 12, g);
+    }
+    const ddv3 r0 = A.lddv(F_X0R, g), v0 = A.lddv(F_X0V, g), gl0 = A.lddv(F_X0GL, g);
+    const v3 w0 = A.ldv(F_X0W, g);
+    A.stddv(F_R, g, add(r0, scale(mkdd(dt), cv)));
+    A.stddv(F_V, g, add(v0, mkddv(dt * ca)));
+    const v3 w = cw / dt_k;
+    ddm3 sm = A.lddm(F_X0S, g);
+    const v3 dw = (2.0 / 3) * (w - w0);
+    sm = add(sm, rotation_matrix(w - dw, dt * dt_k / 2, DevSin{}) * todbl(sm));
+    sm = add(sm, rotation_matrix(w + dw, dt * dt_k / 2, DevSin{}) * todbl(sm));
+    A.stddv(F_GL, g, add(gl0, mkddv(dt * ct)));
+    if (k == 25) {
+        sm = add(sm, rotation_matrix(egrad, 1.0 / 72, DevSin{}) * todbl(sm));
+        sm = orthogonalized(sm);
+    }
+    A.stddm(F_S, g, sm);
+    A.stv(F_W, g, w);
+    const double t = d.t0[d.bsys[g]] + tadd;
+    A.st(F_GM, g, A.ld(F_GM0, g) + A.ld(F_DGM, g) * t);
+    A.st(F_EXJ2, g, A.ld(F_DJ2, g) * t);
+}
+
+__global__ void k_extrema(View d, int invert) {
+    const Acc A{d.F, d.nbp};
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= d.nb) return;
+    if (invert) A.st(F_MIND, g, 1 / A.ld(F_MIND, g));
+    else { A.st(F_MIND, g, 0.0); A.st(F_MAXI, g, 0.0); }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+void set_rk_table(const double *tab650, hipStream_t st) {
+    (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(c_rk), tab650, sizeof(double) * 650, 0, hipMemcpyHostToDevice, st);
+}
+
+void launch_deform(const View &d, const int2 *blk, int nblk, int prepare, hipStream_t st) {
+    if (nblk) hipLaunchKernelGGL(k_deform<kBS>, dim3(nblk), dim3(kBS), 0, st, d, blk, prepare);
+}
+void launch_main_pool(const View &d, const int2 *blk, int nblk, hipStream_t st) {
+    if (nblk) hipLaunchKernelGGL(k_main_pool<kBS>, dim3(nblk), dim3(kBS), 0, st, d, blk);
+}
+void launch_main_serial(const View &d, const int2 *blk, int nblk, hipStream_t st) {
+    if (nblk) hipLaunchKernelGGL(k_main_serial<kBS>, dim3(nblk), dim3(kBS), 0, st, d, blk);
+}
+void launch_np(const View &d, int nrows, int max_n, int maxT, const int2 *recv_blk, int nrecv, hipStream_t st) {
+    if (!nrows) return;
+    hipLaunchKernelGGL(k_np_field, dim3((max_n + 63) / 64, nrows), dim3(64), 0, st, d, nrows);
+    const int nthr = nrows * maxT;
+    hipLaunchKernelGGL(k_np_item, dim3((nthr + 63) / 64), dim3(64), 0, st, d, nrows, maxT);
+    if (nrecv) hipLaunchKernelGGL(k_np_recv, dim3(nrecv), dim3(kBS), 0, st, d, recv_blk, maxT);
+}
+void launch_stage(const View &d, int k, double dt, double dt_k, double tadd, hipStream_t st) {
+    hipLaunchKernelGGL(k_stage, dim3((d.nb + 255) / 256), dim3(256), 0, st, d, k, dt, dt_k, tadd);
+}
+void launch_extrema(const View &d, int invert, hipStream_t st) {
+    hipLaunchKernelGGL(k_extrema, dim3((d.nb + 255) / 256), dim3(256), 0, st, d, invert);
+}
+
+}  // namespace eph

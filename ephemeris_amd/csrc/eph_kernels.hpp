@@ -1,0 +1,37 @@
+// Kernel-side view of a resident batch and the launch entry points.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/ephemeris_hip.h"
+#include "eph_layout.hpp"
+#include "eph_math.hpp"
+
+namespace eph {
+
+constexpr int kBS = 64;          // bodies per workgroup in the j-loop kernels (one wave64)
+
+struct View {
+    double *F;                   // field-major state, F[f * nbp + g]
+    int64_t nbp;                 // field stride
+    int nb;                      // bodies in the batch
+    const SysMeta *sys;
+    const NPItem *items;
+    const eph_geopotential *gps;
+    const eph_ring *rings;
+    const int32_t *np_gp;        // per body: geopotential model or -1
+    const int32_t *np_ring;      // per body: ring model or -1
+    const int32_t *bsys;         // per body: system index
+    double *an;                  // non-point field per (item, partner), 3 doubles
+    double *part;                // per (item, round) partial sums, 6 doubles
+    const double *t0;            // per system: t_eph (hi+lo) at the start of integrate
+};
+
+void set_rk_table(const double *tab650, hipStream_t st);
+void launch_deform(const View &d, const int2 *blk, int nblk, int prepare, hipStream_t st);
+void launch_main_pool(const View &d, const int2 *blk, int nblk, hipStream_t st);
+void launch_main_serial(const View &d, const int2 *blk, int nblk, hipStream_t st);
+void launch_np(const View &d, int nrows, int max_n, int maxT, const int2 *recv_blk, int nrecv, hipStream_t st);
+void launch_stage(const View &d, int k, double dt, double dt_k, double tadd, hipStream_t st);
+void launch_extrema(const View &d, int invert, hipStream_t st);
+
+}  // namespace eph

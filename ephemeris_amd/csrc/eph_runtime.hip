@@ -1,0 +1,523 @@
+// Host runtime of the C ABI declared in include/ephemeris_hip.h: batches of
+// systems resident in HBM, launch plans for both summation orders, the RK12
+// driver loop and HIP-event kernel timing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "eph_kernels.hpp"
+#include "eph_runtime.hpp"
+
+namespace eph {
+
+static thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_CHECK(expr)                                                          \
+    do {                                                                         \
+        hipError_t e_ = (expr);                                                  \
+        if (e_ != hipSuccess)                                                    \
+            return fail(EPH_EDEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+static const double kRK[325][2] = {
+#include "../tables/rk12_feagin.inc"
+};
+
+// host double-double add, src/math/dfloat_t.h:83-89
+static dd host_add(dd x, dd y) { return add(x, y); }
+
+// ---- eph_mass <-> field-major transposition ----------------------------------
+namespace {
+struct FieldMap { int field; size_t offset; };
+#define EM(member) offsetof(eph_mass, member)
+std::vector<FieldMap> mass_fields() {
+    std::vector<FieldMap> m;
+    auto ddv = [&](int f, size_t o) { for (int q = 0; q < 6; ++q) m.push_back({f + q, o + 8 * q}); };
+    auto v = [&](int f, size_t o, int k) { for (int q = 0; q < k; ++q) m.push_back({f + q, o + 8 * q}); };
+    ddv(F_R, EM(r)); ddv(F_V, EM(v)); ddv(F_GL, EM(GL));
+    v(F_S, EM(s), 18); v(F_W, EM(w), 3);
+    v(F_GM, EM(GM), 1); v(F_EXJ2, EM(exJ2), 1);
+    v(F_GM0, EM(GM0), 1); v(F_DGM, EM(dGM), 1); v(F_DJ2, EM(dJ2), 1); v(F_RAD, EM(R), 1);
+    v(F_INERTIA, EM(inertia), 1); v(F_K2, EM(k2), 1); v(F_K2R, EM(k2r), 1); v(F_TD, EM(tide_delay), 1);
+    v(F_TDF, EM(tide_delay_factor), 1); v(F_LUM, EM(lum), 1); v(F_RECPT, EM(recpt), 1);
+    v(F_A, EM(A), 1); v(F_R2, EM(R2), 1); v(F_SR2, EM(sR2), 1); v(F_RR2G4C, EM(rR2G_4c), 1);
+    v(F_CST, EM(C_static), 9);
+    v(F_PHI, EM(phi), 1); v(F_BETA2, EM(beta2), 1);
+    v(F_NACC, EM(naccel), 3); v(F_DACC, EM(daccel), 3); v(F_DTQ, EM(dtorque), 3);
+    v(F_GACC, EM(gaccel), 3); v(F_BETA, EM(beta), 3);
+    v(F_GI, EM(GI), 9); v(F_CPOT, EM(C_potential), 9);
+    v(F_EGRAD, EM(Egrad), 3); v(F_IDACC, EM(idaccel), 3); v(F_IDTQ, EM(idtorque), 3);
+    v(F_JDACC, EM(jdaccel), 3); v(F_JDTQ, EM(jdtorque), 3);
+    v(F_MIND, EM(min_distance), 1); v(F_MAXI, EM(max_influence), 1);
+    return m;
+}
+#undef EM
+const std::vector<FieldMap> &fields() {
+    static const std::vector<FieldMap> f = mass_fields();
+    return f;
+}
+constexpr int kMassFieldEnd = F_X0R;   // fields [0, F_X0R) carry eph_mass data
+}  // namespace
+
+// ---- profiling --------------------------------------------------------------
+const char *const kKernelNames[kNumKernelClasses] = {
+    "stage_update", "deform_spin", "main_force_pool", "main_force_serial", "nonpoint"};
+
+void Profiler::begin(int cls, hipStream_t st) {
+    if (!enabled) return;
+    Rec r;
+    r.cls = cls;
+    if (free_ev.size() < 2) {
+        hipEvent_t a, b;
+        (void)hipEventCreate(&a);
+        (void)hipEventCreate(&b);
+        free_ev.push_back(a);
+        free_ev.push_back(b);
+    }
+    r.a = free_ev.back(); free_ev.pop_back();
+    r.b = free_ev.back(); free_ev.pop_back();
+    (void)hipEventRecord(r.a, st);
+    open.push_back(r);
+}
+void Profiler::end(hipStream_t st) {
+    if (!enabled || open.empty()) return;
+    (void)hipEventRecord(open.back().b, st);
+    pending.push_back(open.back());
+    open.pop_back();
+}
+void Profiler::collect() {
+    for (auto &r : pending) {
+        float ms = 0;
+        (void)hipEventSynchronize(r.b);
+        (void)hipEventElapsedTime(&ms, r.a, r.b);
+        total_ms[r.cls] += ms;
+        count[r.cls] += 1;
+        free_ev.push_back(r.a);
+        free_ev.push_back(r.b);
+    }
+    pending.clear();
+}
+void Profiler::reset() {
+    collect();
+    for (int i = 0; i < kNumKernelClasses; ++i) { total_ms[i] = 0; count[i] = 0; }
+}
+Profiler::~Profiler() {
+    for (auto e : free_ev) (void)hipEventDestroy(e);
+}
+
+struct Scope {
+    Profiler &p;
+    hipStream_t st;
+    Scope(Profiler &p_, int cls, hipStream_t s) : p(p_), st(s) { p.begin(cls, st); }
+    ~Scope() { p.end(st); }
+};
+
+// ---- system ------------------------------------------------------------------
+Batch::~Batch() {
+    profiler.collect();
+    for (void *p : dev_allocs) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+template <typename T>
+static int dev_upload(Batch &b, T *&dst, const std::vector<T> &src) {
+    dst = nullptr;
+    if (src.empty()) return EPH_OK;
+    HIP_CHECK(hipMalloc(&dst, sizeof(T) * src.size()));
+    b.dev_allocs.push_back(dst);
+    HIP_CHECK(hipMemcpy(dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice));
+    return EPH_OK;
+}
+
+static void append_blocks(std::vector<int2> &blk, int s, int n) {
+    for (int i0 = 0; i0 < n; i0 += kBS) blk.push_back(make_int2(s, i0));
+}
+
+int Batch::build_plan(int mode) {
+    Plan &pl = plan[mode];
+    std::vector<SysMeta> meta(n_sys);
+    std::vector<NPItem> items;
+    std::vector<int2> blk_pool, blk_serial, blk_recv;
+    int64_t an_rows = 0;
+    pl.maxT = 1;
+    pl.max_n = 1;
+    for (int64_t s = 0; s < n_sys; ++s) {
+        SysMeta &m = meta[s];
+        const int64_t n = sys_n[s];
+        m.off = (int32_t)sys_off[s];
+        m.n = (int32_t)n;
+        m.item0 = (int32_t)items.size();
+        m.nitem = 0;
+        m.T = 0;
+        pl.max_n = std::max<int>(pl.max_n, (int)n);
+        // msystem::accel task count, mass.cpp:226-231
+        int64_t sq = (n * n + 2303) / 2304;
+        int64_t n_task = std::min<int64_t>(n, sq);
+        m.serial = (mode == 1 || n_task <= 1) ? 1 : 0;
+        if (m.serial) {
+            append_blocks(blk_serial, (int)s, (int)n);
+            continue;
+        }
+        n_task = std::min<int64_t>(n_task, 64);
+        // items in creation order: last body first, geopotential before ring
+        std::vector<NPItem> sys_items;
+        for (int64_t i = n; i > 0;) {
+            --i;
+            const int64_t g = sys_off[s] + i;
+            if (host_gp[g] >= 0) sys_items.push_back(NPItem{(int32_t)g, 0, host_gp[g], 0, (int32_t)s, 0});
+            if (host_ring[g] >= 0) sys_items.push_back(NPItem{(int32_t)g, 1, host_ring[g], 0, (int32_t)s, 0});
+        }
+        const int64_t nnp = (int64_t)sys_items.size();
+        const int64_t T = std::min(nnp, n_task);
+        for (int64_t q = 0; q < nnp; ++q) {
+            // chain c holds items [c*nnp/T, (c+1)*nnp/T)
+            int64_t c = 0;
+            while ((c + 1) * nnp / T <= q) ++c;
+            sys_items[q].chain = (int32_t)c;
+            sys_items[q].an_off = (int32_t)an_rows;
+            an_rows += n;
+        }
+        m.nitem = (int32_t)nnp;
+        m.T = (int32_t)T;
+        pl.maxT = std::max<int>(pl.maxT, (int)T);
+        items.insert(items.end(), sys_items.begin(), sys_items.end());
+        append_blocks(blk_pool, (int)s, (int)n);
+        if (nnp) append_blocks(blk_recv, (int)s, (int)n);
+    }
+    int rc;
+    if ((rc = dev_upload(*this, pl.d_sys, meta))) return rc;
+    if ((rc = dev_upload(*this, pl.d_items, items))) return rc;
+    if ((rc = dev_upload(*this, pl.d_blk_pool, blk_pool))) return rc;
+    if ((rc = dev_upload(*this, pl.d_blk_serial, blk_serial))) return rc;
+    if ((rc = dev_upload(*this, pl.d_blk_recv, blk_recv))) return rc;
+    pl.n_pool = (int)blk_pool.size();
+    pl.n_serial = (int)blk_serial.size();
+    pl.n_recv = (int)blk_recv.size();
+    pl.nrows = (int)items.size();
+    pl.an_rows = an_rows;
+    if (an_rows) {
+        HIP_CHECK(hipMalloc(&pl.d_an, sizeof(double) * 3 * an_rows));
+        dev_allocs.push_back(pl.d_an);
+        HIP_CHECK(hipMalloc(&pl.d_part, sizeof(double) * 6 * (size_t)pl.nrows * pl.maxT));
+        dev_allocs.push_back(pl.d_part);
+    }
+    return EPH_OK;
+}
+
+View Batch::view(int mode) const {
+    const Plan &pl = plan[mode];
+    View v;
+    v.F = F;
+    v.nbp = nbp;
+    v.nb = (int)nb;
+    v.sys = pl.d_sys;
+    v.items = pl.d_items;
+    v.gps = d_gps;
+    v.rings = d_rings;
+    v.np_gp = d_np_gp;
+    v.np_ring = d_np_ring;
+    v.bsys = d_bsys;
+    v.an = pl.d_an;
+    v.part = pl.d_part;
+    v.t0 = d_t0;
+    return v;
+}
+
+int Batch::accel(int mode, bool prepare_only) {
+    const View v = view(mode);
+    const Plan &pl = plan[mode];
+    {
+        Scope sc(profiler, 1, stream);
+        launch_deform(v, d_blk_all, n_blk_all, prepare_only ? 0 : 1, stream);
+    }
+    if (prepare_only) return EPH_OK;
+    if (pl.n_pool) {
+        Scope sc(profiler, 2, stream);
+        launch_main_pool(v, pl.d_blk_pool, pl.n_pool, stream);
+    }
+    if (pl.n_serial) {
+        Scope sc(profiler, 3, stream);
+        launch_main_serial(v, pl.d_blk_serial, pl.n_serial, stream);
+    }
+    if (pl.nrows) {
+        Scope sc(profiler, 4, stream);
+        launch_np(v, pl.nrows, pl.max_n, pl.maxT, pl.d_blk_recv, pl.n_recv, stream);
+    }
+    return EPH_OK;
+}
+
+int Batch::integrate(double dt, int64_t n_step, int mode) {
+    if (n_step == 0) return EPH_OK;
+    if (n_step < 0) { n_step = -n_step; dt = -dt; }
+    std::vector<double> t0(n_sys);
+    for (int64_t s = 0; s < n_sys; ++s) t0[s] = todbl(t_eph[s]);
+    HIP_CHECK(hipMemcpyAsync(d_t0, t0.data(), sizeof(double) * n_sys, hipMemcpyHostToDevice, stream));
+    const View v = view(mode);
+    launch_extrema(v, 0, stream);
+    for (int64_t i_step = 0; i_step < n_step; ++i_step) {
+        for (int k = 1; k <= 25; ++k) {
+            double dt_k = 0;                                   // RungeKutta.impl:148-152
+            for (int j = 0; j < k; ++j) {
+                const double *c = kRK[k * (k - 1) / 2 + j];
+                if (c[0] != 0) dt_k += c[0] + c[1];
+            }
+            const double tadd = dt * (dt_k + (double)i_step);
+            {
+                Scope sc(profiler, 0, stream);
+                launch_stage(v, k, dt, dt_k, tadd, stream);
+            }
+            int rc = accel(mode, false);
+            if (rc) return rc;
+        }
+        if (profiler.enabled && (i_step & 15) == 15) profiler.collect();
+    }
+    launch_extrema(v, 1, stream);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(stream));
+    for (int64_t s = 0; s < n_sys; ++s) t_eph[s] = host_add(t_eph[s], mkdd(dt * (double)n_step));
+    return EPH_OK;
+}
+
+int Batch::upload(const eph_mass *masses) {
+    std::vector<double> buf((size_t)kMassFieldEnd * nbp, 0.0);
+    for (int64_t g = 0; g < nb; ++g) {
+        const char *m = (const char *)&masses[g];
+        for (const auto &f : fields()) {
+            double x;
+            memcpy(&x, m + f.offset, 8);
+            buf[(size_t)f.field * nbp + g] = x;
+        }
+        buf[(size_t)F_PHIC * nbp + g] = masses[g].phi / kC2;
+        buf[(size_t)F_NACCC * nbp + g] = masses[g].naccel.x * (1 / kC2);
+        buf[(size_t)(F_NACCC + 1) * nbp + g] = masses[g].naccel.y * (1 / kC2);
+        buf[(size_t)(F_NACCC + 2) * nbp + g] = masses[g].naccel.z * (1 / kC2);
+    }
+    HIP_CHECK(hipMemcpy(F, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice));
+    sid.resize(nb);
+    for (int64_t g = 0; g < nb; ++g) sid[g] = masses[g].sid;
+    return EPH_OK;
+}
+
+int Batch::download(eph_mass *masses) {
+    std::vector<double> buf((size_t)kMassFieldEnd * nbp);
+    HIP_CHECK(hipStreamSynchronize(stream));
+    HIP_CHECK(hipMemcpy(buf.data(), F, sizeof(double) * buf.size(), hipMemcpyDeviceToHost));
+    for (int64_t g = 0; g < nb; ++g) {
+        char *m = (char *)&masses[g];
+        memset(m, 0, sizeof(eph_mass));
+        for (const auto &f : fields()) {
+            double x = buf[(size_t)f.field * nbp + g];
+            memcpy(m + f.offset, &x, 8);
+        }
+        masses[g].sid = sid[g];
+        masses[g].gp = host_gp_global[g];
+        masses[g].ring = host_ring_global[g];
+    }
+    return EPH_OK;
+}
+
+int create_batch(int64_t n_sys, const int64_t *sys_n, const eph_mass *masses,
+                 const eph_geopotential *gps, int64_t n_gp, const eph_ring *rings, int64_t n_ring,
+                 const double *t_eph, Batch **out) {
+    if (!out || n_sys <= 0 || !sys_n || !masses) return fail(EPH_EINVAL, "eph_system_create: null or empty input");
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    auto b = new Batch();
+    b->device = dev;
+    b->n_sys = n_sys;
+    b->sys_n.assign(sys_n, sys_n + n_sys);
+    b->sys_off.resize(n_sys + 1, 0);
+    for (int64_t s = 0; s < n_sys; ++s) {
+        if (sys_n[s] <= 0 || sys_n[s] > (1 << 24)) { delete b; return fail(EPH_EINVAL, "system size out of range"); }
+        b->sys_off[s + 1] = b->sys_off[s] + sys_n[s];
+    }
+    b->nb = b->sys_off[n_sys];
+    b->nbp = (b->nb + 63) / 64 * 64;
+    b->t_eph.resize(n_sys);
+    for (int64_t s = 0; s < n_sys; ++s) b->t_eph[s] = t_eph ? dd{t_eph[2 * s], t_eph[2 * s + 1]} : dd{0, 0};
+    b->gps.assign(gps, gps + (gps ? n_gp : 0));
+    b->rings.assign(rings, rings + (rings ? n_ring : 0));
+    for (const auto &g : b->gps)
+        if (g.Nz < 2 || g.Nz > EPH_GP_MAX_DEGREE || g.Nt < 0 || g.Nt > g.Nz) { delete b; return fail(EPH_EINVAL, "bad geopotential degree"); }
+    for (const auto &r : b->rings)
+        if (r.N < 0 || r.N > EPH_RING_MAX_DISKS) { delete b; return fail(EPH_EINVAL, "bad ring disk count"); }
+    b->host_gp.resize(b->nb);
+    b->host_ring.resize(b->nb);
+    b->host_gp_global.resize(b->nb);
+    b->host_ring_global.resize(b->nb);
+    std::vector<int32_t> bsys(b->nb);
+    for (int64_t s = 0; s < n_sys; ++s)
+        for (int64_t g = b->sys_off[s]; g < b->sys_off[s + 1]; ++g) bsys[g] = (int32_t)s;
+    for (int64_t g = 0; g < b->nb; ++g) {
+        const int64_t gp = masses[g].gp, rg = masses[g].ring;
+        if (gp >= n_gp || rg >= n_ring) { delete b; return fail(EPH_EINVAL, "model index out of range"); }
+        b->host_gp[g] = gp >= 0 ? (int32_t)gp : -1;
+        b->host_ring[g] = rg >= 0 ? (int32_t)rg : -1;
+        b->host_gp_global[g] = gp >= 0 ? gp : -1;
+        b->host_ring_global[g] = rg >= 0 ? rg : -1;
+    }
+    int rc;
+    auto bail = [&](int code) { delete b; return code; };
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(EPH_EDEVICE, "hipStreamCreate failed"));
+    size_t bytes = sizeof(double) * (size_t)F_COUNT * b->nbp;
+    if (hipMalloc(&b->F, bytes) != hipSuccess) return bail(fail(EPH_ENOMEM, "hipMalloc of the state failed"));
+    b->dev_allocs.push_back(b->F);
+    if (hipMemset(b->F, 0, bytes) != hipSuccess) return bail(fail(EPH_EDEVICE, "hipMemset failed"));
+    if ((rc = dev_upload(*b, b->d_gps, b->gps))) return bail(rc);
+    if ((rc = dev_upload(*b, b->d_rings, b->rings))) return bail(rc);
+    if ((rc = dev_upload(*b, b->d_np_gp, b->host_gp))) return bail(rc);
+    if ((rc = dev_upload(*b, b->d_np_ring, b->host_ring))) return bail(rc);
+    if ((rc = dev_upload(*b, b->d_bsys, bsys))) return bail(rc);
+    std::vector<double> t0(n_sys, 0.0);
+    if ((rc = dev_upload(*b, b->d_t0, t0))) return bail(rc);
+    std::vector<int2> blk_all;
+    for (int64_t s = 0; s < n_sys; ++s) append_blocks(blk_all, (int)s, (int)sys_n[s]);
+    if ((rc = dev_upload(*b, b->d_blk_all, blk_all))) return bail(rc);
+    b->n_blk_all = (int)blk_all.size();
+    if ((rc = b->build_plan(0))) return bail(rc);
+    if ((rc = b->build_plan(1))) return bail(rc);
+    set_rk_table(&kRK[0][0], b->stream);
+    if ((rc = b->upload(masses))) return bail(rc);
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return bail(fail(EPH_EDEVICE, "stream sync failed"));
+    *out = b;
+    return EPH_OK;
+}
+
+}  // namespace eph
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+using eph::Batch;
+using eph::fail;
+
+// eph_system is the opaque C name of a Batch
+static Batch *B(eph_system *s) { return reinterpret_cast<Batch *>(s); }
+static const Batch *B(const eph_system *s) { return reinterpret_cast<const Batch *>(s); }
+
+extern "C" {
+
+const char *eph_last_error(void) { return eph::g_last_error.c_str(); }
+int eph_abi_version(void) { return EPH_ABI_VERSION; }
+
+int eph_init(int device) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(EPH_ENODEV, "no HIP device visible");
+    if (device < 0 || device >= n) return fail(EPH_ENODEV, "device index out of range");
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, device) != hipSuccess) return fail(EPH_ENODEV, "hipGetDeviceProperties failed");
+    if (strncmp(p.gcnArchName, "gfx950", 6) != 0)
+        return fail(EPH_ENODEV, std::string("library built for gfx950, device is ") + p.gcnArchName);
+    if (hipSetDevice(device) != hipSuccess) return fail(EPH_ENODEV, "hipSetDevice failed");
+    return EPH_OK;
+}
+
+int eph_system_create(int64_t n_sys, const int64_t *sys_n, const eph_mass *masses,
+                      const eph_geopotential *gps, int64_t n_gp, const eph_ring *rings, int64_t n_ring,
+                      const double *t_eph, eph_system **out) {
+    Batch *b = nullptr;
+    int rc = eph::create_batch(n_sys, sys_n, masses, gps, n_gp, rings, n_ring, t_eph, &b);
+    if (rc) return rc;
+    *out = reinterpret_cast<eph_system *>(b);
+    return EPH_OK;
+}
+
+int eph_system_destroy(eph_system *sys) {
+    if (!sys) return fail(EPH_EINVAL, "null system");
+    delete B(sys);
+    return EPH_OK;
+}
+
+int eph_system_upload(eph_system *sys, const eph_mass *masses, const double *t_eph) {
+    if (!sys || !masses) return fail(EPH_EINVAL, "null argument");
+    int rc = B(sys)->upload(masses);
+    if (rc) return rc;
+    if (t_eph)
+        for (int64_t s = 0; s < B(sys)->n_sys; ++s) B(sys)->t_eph[s] = eph::dd{t_eph[2 * s], t_eph[2 * s + 1]};
+    return EPH_OK;
+}
+
+int eph_system_download(eph_system *sys, eph_mass *masses, double *t_eph) {
+    if (!sys || !masses) return fail(EPH_EINVAL, "null argument");
+    int rc = B(sys)->download(masses);
+    if (rc) return rc;
+    if (t_eph)
+        for (int64_t s = 0; s < B(sys)->n_sys; ++s) { t_eph[2 * s] = B(sys)->t_eph[s].hi; t_eph[2 * s + 1] = B(sys)->t_eph[s].lo; }
+    return EPH_OK;
+}
+
+int64_t eph_system_bodies(const eph_system *sys) { return sys ? B(sys)->nb : -1; }
+int64_t eph_system_count(const eph_system *sys) { return sys ? B(sys)->n_sys : -1; }
+
+int eph_deform(eph_system *sys) {
+    if (!sys) return fail(EPH_EINVAL, "null system");
+    int rc = B(sys)->accel(0, true);
+    if (rc) return rc;
+    if (hipStreamSynchronize(B(sys)->stream) != hipSuccess) return fail(EPH_EDEVICE, "deform failed");
+    return EPH_OK;
+}
+
+int eph_accel(eph_system *sys, int summation) {
+    if (!sys) return fail(EPH_EINVAL, "null system");
+    if (summation != EPH_SUMMATION_POOL && summation != EPH_SUMMATION_SERIAL)
+        return fail(EPH_EINVAL, "summation must be EPH_SUMMATION_POOL or EPH_SUMMATION_SERIAL");
+    int rc = B(sys)->accel(summation == EPH_SUMMATION_SERIAL ? 1 : 0, false);
+    if (rc) return rc;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(EPH_EDEVICE, std::string("accel launch: ") + hipGetErrorString(e));
+    if (hipStreamSynchronize(B(sys)->stream) != hipSuccess) return fail(EPH_EDEVICE, "accel failed");
+    return EPH_OK;
+}
+
+int eph_integrate_ex(eph_system *sys, double dt, int64_t n_step, int summation) {
+    if (!sys) return fail(EPH_EINVAL, "null system");
+    if (!(dt == dt) || dt == 0) return fail(EPH_EINVAL, "dt must be finite and non-zero");
+    if (summation != EPH_SUMMATION_POOL && summation != EPH_SUMMATION_SERIAL)
+        return fail(EPH_EINVAL, "summation must be EPH_SUMMATION_POOL or EPH_SUMMATION_SERIAL");
+    return B(sys)->integrate(dt, n_step, summation == EPH_SUMMATION_SERIAL ? 1 : 0);
+}
+
+int eph_integrate(eph_system *sys, double dt, int64_t n_step) {
+    return eph_integrate_ex(sys, dt, n_step, EPH_SUMMATION_POOL);
+}
+
+int eph_profile_enable(eph_system *sys, int enable) {
+    if (!sys) return fail(EPH_EINVAL, "null system");
+    B(sys)->profiler.collect();
+    B(sys)->profiler.enabled = enable != 0;
+    return EPH_OK;
+}
+
+int eph_profile_reset(eph_system *sys) {
+    if (!sys) return fail(EPH_EINVAL, "null system");
+    (void)hipStreamSynchronize(B(sys)->stream);
+    B(sys)->profiler.reset();
+    return EPH_OK;
+}
+
+int eph_profile_read(eph_system *sys, const char **names, double *avg_ms, int64_t *launches, int cap) {
+    if (!sys) return fail(EPH_EINVAL, "null system");
+    (void)hipStreamSynchronize(B(sys)->stream);
+    auto &p = B(sys)->profiler;
+    p.collect();
+    int n = std::min(cap, eph::kNumKernelClasses);
+    for (int i = 0; i < n; ++i) {
+        if (names) names[i] = eph::kKernelNames[i];
+        if (avg_ms) avg_ms[i] = p.count[i] ? p.total_ms[i] / (double)p.count[i] : 0.0;
+        if (launches) launches[i] = p.count[i];
+    }
+    return n;
+}
+
+}  // extern "C"

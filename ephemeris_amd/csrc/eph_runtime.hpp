@@ -1,0 +1,79 @@
+// Host-side representation of a resident batch (internal to the library).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "eph_kernels.hpp"
+
+namespace eph {
+
+int fail(int code, const std::string &msg);
+
+constexpr int kNumKernelClasses = 5;
+extern const char *const kKernelNames[kNumKernelClasses];
+
+// Per-kernel-class device time from HIP events recorded on the batch stream.
+struct Profiler {
+    struct Rec { int cls; hipEvent_t a, b; };
+    bool enabled = false;
+    std::vector<Rec> open, pending;
+    std::vector<hipEvent_t> free_ev;
+    double total_ms[kNumKernelClasses] = {};
+    int64_t count[kNumKernelClasses] = {};
+    void begin(int cls, hipStream_t st);
+    void end(hipStream_t st);
+    void collect();
+    void reset();
+    ~Profiler();
+};
+
+// Launch plan for one summation order (0 = pool, 1 = serial).
+struct Plan {
+    SysMeta *d_sys = nullptr;
+    NPItem *d_items = nullptr;
+    int2 *d_blk_pool = nullptr, *d_blk_serial = nullptr, *d_blk_recv = nullptr;
+    int n_pool = 0, n_serial = 0, n_recv = 0;
+    int nrows = 0, maxT = 1, max_n = 1;
+    int64_t an_rows = 0;
+    double *d_an = nullptr, *d_part = nullptr;
+};
+
+struct Batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t n_sys = 0, nb = 0, nbp = 0;
+    std::vector<int64_t> sys_n, sys_off;
+    std::vector<eph_geopotential> gps;
+    std::vector<eph_ring> rings;
+    std::vector<dd> t_eph;
+    std::vector<uint64_t> sid;
+    std::vector<int32_t> host_gp, host_ring;
+    std::vector<int64_t> host_gp_global, host_ring_global;
+
+    double *F = nullptr;
+    eph_geopotential *d_gps = nullptr;
+    eph_ring *d_rings = nullptr;
+    int32_t *d_np_gp = nullptr, *d_np_ring = nullptr, *d_bsys = nullptr;
+    double *d_t0 = nullptr;
+    int2 *d_blk_all = nullptr;
+    int n_blk_all = 0;
+    Plan plan[2];
+    std::vector<void *> dev_allocs;
+    Profiler profiler;
+
+    ~Batch();
+    int build_plan(int mode);
+    View view(int mode) const;
+    int accel(int mode, bool deform_only);
+    int integrate(double dt, int64_t n_step, int mode);
+    int upload(const eph_mass *masses);
+    int download(eph_mass *masses);
+};
+
+int create_batch(int64_t n_sys, const int64_t *sys_n, const eph_mass *masses,
+                 const eph_geopotential *gps, int64_t n_gp, const eph_ring *rings, int64_t n_ring,
+                 const double *t_eph, Batch **out);
+
+}  // namespace eph

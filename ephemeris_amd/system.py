@@ -1,0 +1,87 @@
+"""Python handle on a batch of systems resident in HBM (eph_system)."""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Sequence
+
+import numpy as np
+
+from . import _lib
+from .layout import GP_DTYPE, MASS_DTYPE, RING_DTYPE
+
+
+def _ptr(a):
+    return None if a is None or len(a) == 0 else a.ctypes.data_as(C.c_void_p)
+
+
+class GpuSystems:
+    """One or more independent systems integrated together on the GPU.
+
+    ``masses`` holds the bodies of all systems back to back (``sys_n`` gives
+    the count of each); model indices in ``masses['gp']``/``['ring']`` index
+    ``gps``/``rings``.  ``t_eph`` is an (n_sys, 2) array of double-double
+    epochs."""
+
+    def __init__(self, masses: np.ndarray, sys_n: Sequence[int] = None, gps=None, rings=None,
+                 t_eph=None, device=0):
+        _lib.init(device)
+        masses = np.ascontiguousarray(masses, dtype=MASS_DTYPE)
+        sys_n = np.ascontiguousarray([len(masses)] if sys_n is None else sys_n, dtype=np.int64)
+        gps = np.ascontiguousarray(np.zeros(0, GP_DTYPE) if gps is None else gps, dtype=GP_DTYPE)
+        rings = np.ascontiguousarray(np.zeros(0, RING_DTYPE) if rings is None else rings, dtype=RING_DTYPE)
+        if t_eph is None:
+            t_eph = np.zeros((len(sys_n), 2))
+        t_eph = np.ascontiguousarray(t_eph, dtype=np.float64).reshape(len(sys_n), 2)
+        if int(sys_n.sum()) != len(masses):
+            raise ValueError("sum(sys_n) must equal len(masses)")
+        h = C.c_void_p()
+        _lib.check(_lib.lib().eph_system_create(len(sys_n), _ptr(sys_n), _ptr(masses), _ptr(gps), len(gps),
+                                                _ptr(rings), len(rings), _ptr(t_eph), C.byref(h)))
+        self.h = h
+        self.sys_n = sys_n
+        self.n_sys = len(sys_n)
+        self.nb = len(masses)
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.lib().eph_system_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def download(self):
+        m = np.zeros(self.nb, MASS_DTYPE)
+        t = np.zeros((self.n_sys, 2))
+        _lib.check(_lib.lib().eph_system_download(self.h, _ptr(m), _ptr(t)))
+        return m, t
+
+    def upload(self, masses, t_eph=None):
+        masses = np.ascontiguousarray(masses, dtype=MASS_DTYPE)
+        t = None if t_eph is None else np.ascontiguousarray(t_eph, dtype=np.float64)
+        _lib.check(_lib.lib().eph_system_upload(self.h, _ptr(masses), _ptr(t)))
+
+    def deform(self):
+        _lib.check(_lib.lib().eph_deform(self.h))
+
+    def accel(self, summation=_lib.SUMMATION_POOL):
+        _lib.check(_lib.lib().eph_accel(self.h, summation))
+
+    def integrate(self, dt, n_step, summation=_lib.SUMMATION_POOL):
+        _lib.check(_lib.lib().eph_integrate_ex(self.h, float(dt), int(n_step), summation))
+
+    def profile(self, enable=True):
+        _lib.check(_lib.lib().eph_profile_enable(self.h, int(enable)))
+
+    def profile_reset(self):
+        _lib.check(_lib.lib().eph_profile_reset(self.h))
+
+    def profile_read(self):
+        names = (C.c_char_p * 16)()
+        avg = (C.c_double * 16)()
+        cnt = (C.c_int64 * 16)()
+        n = _lib.check(_lib.lib().eph_profile_read(self.h, names, avg, cnt, 16))
+        return {names[i].decode(): (avg[i], cnt[i]) for i in range(n)}

@@ -1,0 +1,138 @@
+/*
+ * ephemeris_hip.h -- C ABI of the MI355X (gfx950) force-evaluation and RK12
+ * back end for Ephemeris.
+ *
+ * The library keeps systems resident in HBM.  One eph_system holds one or
+ * more independent gravitating systems (a batch); every call computes all of
+ * them.  All entry points return EPH_OK (0) or a negative status and set a
+ * thread-local message readable with eph_last_error().  Nothing calls exit().
+ * Calls on one eph_system must not overlap; different eph_system objects may
+ * be used from different threads.
+ *
+ * Arithmetic reproduces the reference CPU path (double-double state, FP64
+ * pair terms, no contraction) in the reference's summation order, so results
+ * agree with it bit for bit apart from the last-ulp behaviour of libm
+ * transcendentals (sin in the spin update; atan2/log/sin/cos in ring forces).
+ *
+ * Reference interfaces replaced (paths relative to the reference checkout):
+ *   eph_accel       <- msystem::Cuda_accel()        src/integrators/RungeKutta.cu:418-438
+ *                      msystem::accel(int)          src/physics/mass.cpp:216-397
+ *   eph_integrate   <- msystem::Cuda_RungeKutta12() src/integrators/RungeKutta.cu:391-412
+ *                      msystem::integrate(dt,n,1)   src/physics/mass.cpp:399-442
+ *   eph_deform      <- msystem::deform()            src/physics/mass.cpp:104-110
+ *   eph_system_*    <- the AoS upload/download of cuda_rungekutta_kernel_config::load/save,
+ *                      src/integrators/RungeKutta.cu:77-133 (here the state stays resident)
+ */
+#ifndef EPHEMERIS_HIP_H
+#define EPHEMERIS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EPH_ABI_VERSION 1
+
+enum {
+    EPH_OK = 0,
+    EPH_EINVAL = -1,      /* bad argument */
+    EPH_ENODEV = -2,      /* no usable gfx950 device */
+    EPH_ENOMEM = -3,      /* device or host allocation failed */
+    EPH_EDEVICE = -4,     /* HIP runtime error */
+    EPH_EIO = -5,         /* file not found / parse error */
+    EPH_ESANITY = -6,     /* a body failed the loader sanity checks */
+    EPH_ENOTSUP = -7,     /* requested back end not provided */
+    EPH_EBUSY = -8        /* server: not enough free memory for a job now */
+};
+
+/* Summation order of one force evaluation (msystem::accel, mass.cpp:226-230):
+ * POOL follows the thread-pool branch (taken when a pool exists and
+ * min(N, ceil(N^2/2304)) > 1), SERIAL the single-thread branch. */
+enum { EPH_SUMMATION_POOL = 0, EPH_SUMMATION_SERIAL = -1 };
+
+/* One body: field-for-field mirror of class mass (src/physics/mass.h:67-143)
+ * with double-double values as {hi, lo} and the model pointers replaced by
+ * indices into the model arrays passed at creation (-1 = none).  Every field
+ * is 8 bytes wide. */
+typedef struct { double hi, lo; } eph_dd;
+typedef struct { double x, y, z; } eph_vec;
+typedef struct { eph_dd x, y, z; } eph_ddvec;
+typedef struct { eph_vec x, y, z; } eph_mat;        /* three axes */
+typedef struct { eph_ddvec x, y, z; } eph_ddmat;
+
+typedef struct {
+    eph_ddvec r, v, GL;                 /* position, velocity, G*L/GM */
+    eph_ddmat s;                        /* body frame */
+    eph_vec w;                          /* angular velocity */
+    double GM, exJ2;                    /* time-variable parameters */
+    uint64_t sid;                       /* id, up to 7 chars */
+    double GM0, dGM, dJ2, R, inertia, k2, k2r, tide_delay, tide_delay_factor;
+    double lum, recpt;
+    double A, R2, sR2, rR2G_4c;         /* derived constants */
+    eph_mat C_static;
+    int64_t gp, ring;                   /* model indices, -1 = none */
+    double phi, beta2;                  /* outputs of accel from here on */
+    eph_vec naccel, daccel, dtorque, gaccel, beta;
+    eph_mat GI, C_potential;
+    eph_vec Egrad, idaccel, idtorque, jdaccel, jdtorque;
+    double min_distance, max_influence;
+} eph_mass;
+
+/* Zonal/tesseral geopotential of degree 3..8 (src/physics/geopotential.h:4-38):
+ * c[n*n-4] = J_n, c[n*n-4+2m-1] = C_nm, c[n*n-4+2m] = S_nm. */
+#define EPH_GP_MAX_DEGREE 8
+#define EPH_GP_TABLE 77
+typedef struct { int64_t Nz, Nt; double c[EPH_GP_TABLE]; } eph_geopotential;
+
+/* Ring of disks (src/physics/ring.h:4-28). */
+#define EPH_RING_MAX_DISKS 64
+typedef struct {
+    int64_t N;
+    double GM_ratio, A, J2, GL_R2;
+    double GsR2[EPH_RING_MAX_DISKS], R_R[EPH_RING_MAX_DISKS], H_R[EPH_RING_MAX_DISKS];
+} eph_ring;
+
+typedef struct eph_system eph_system;
+
+/* ---- library ---------------------------------------------------------- */
+int eph_init(int device);                     /* select and check a gfx950 device */
+const char *eph_last_error(void);
+int eph_abi_version(void);
+
+/* ---- systems resident in HBM ---------------------------------------------
+ * A batch of n_sys systems: system k has sys_n[k] bodies, stored
+ * consecutively in `masses`; model indices are global into gps/rings. */
+int eph_system_create(int64_t n_sys, const int64_t *sys_n, const eph_mass *masses,
+                      const eph_geopotential *gps, int64_t n_gp,
+                      const eph_ring *rings, int64_t n_ring,
+                      const double *t_eph /* n_sys x {hi,lo} */, eph_system **out);
+int eph_system_destroy(eph_system *sys);
+int eph_system_upload(eph_system *sys, const eph_mass *masses, const double *t_eph);
+int eph_system_download(eph_system *sys, eph_mass *masses, double *t_eph);
+int64_t eph_system_bodies(const eph_system *sys);
+int64_t eph_system_count(const eph_system *sys);
+
+/* ---- hot path ------------------------------------------------------------ */
+int eph_deform(eph_system *sys);
+int eph_accel(eph_system *sys, int summation);
+/* msystem::integrate(dt, n_step, USE_GPU=1) for every system of the batch:
+ * resets the running extrema, runs n_step RK12 steps with POOL summation,
+ * inverts min_distance and advances t_eph by dt*n_step. */
+int eph_integrate(eph_system *sys, double dt, int64_t n_step);
+/* As eph_integrate but with an explicit summation order for every accel. */
+int eph_integrate_ex(eph_system *sys, double dt, int64_t n_step, int summation);
+
+/* ---- measurement ---------------------------------------------------------- */
+/* Average device time (ms) per launch of each kernel class, measured with
+ * HIP events on the library's stream since the last reset; names[] gets
+ * static strings.  Returns the number of classes written. */
+int eph_profile_enable(eph_system *sys, int enable);
+int eph_profile_read(eph_system *sys, const char **names, double *avg_ms, int64_t *launches, int cap);
+int eph_profile_reset(eph_system *sys);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

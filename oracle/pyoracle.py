@@ -150,6 +150,20 @@ class OSystem:
     def integrate(self, dt, n_step, pool=True):
         lib().o_integrate(self.h, float(dt), int(n_step), int(pool))
 
+    def raw_models(self):
+        """(gp pointer, n_gp, ring pointer, n_ring) of the o_system."""
+        p = C.cast(self.h, C.POINTER(C.c_int64))
+        return p[7], p[5], p[8], p[6]
+
+    def models(self):
+        """Copies of the geopotential and ring tables (GP_DTYPE, RING_DTYPE)."""
+        gp, ngp, rg, nrg = self.raw_models()
+        gps = (np.frombuffer((C.c_char * (GP_DTYPE.itemsize * ngp)).from_address(gp), GP_DTYPE).copy()
+               if ngp else np.zeros(0, GP_DTYPE))
+        rings = (np.frombuffer((C.c_char * (RING_DTYPE.itemsize * nrg)).from_address(rg), RING_DTYPE).copy()
+                 if nrg else np.zeros(0, RING_DTYPE))
+        return gps, rings
+
     def t_eph(self):
         p = C.cast(self.h, C.POINTER(C.c_double))
         # o_system: int64 n; o_mass* m; o_dd t_eph ...
