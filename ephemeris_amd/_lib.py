@@ -34,6 +34,16 @@ SIGNATURES = {
     "eph_profile_enable": (I, [P, I]),
     "eph_profile_read": (I, [P, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.POINTER(C.c_int64), I]),
     "eph_profile_reset": (I, [P]),
+    "eph_msystem_load": (I, [C.c_char_p, I, PP]),
+    "eph_msystem_destroy": (I, [P]),
+    "eph_msystem_size": (I64, [P]),
+    "eph_msystem_masses": (I, [P, P, I64]),
+    "eph_msystem_models": (I, [P, P, I64, P, I64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "eph_msystem_time": (I, [P, C.POINTER(C.c_double)]),
+    "eph_msystem_accel": (I, [P, I]),
+    "eph_msystem_integrate": (I, [P, D, I64, I]),
+    "eph_msystem_diagnostics": (I, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double), I64]),
 }
 
 

@@ -85,3 +85,63 @@ class GpuSystems:
         cnt = (C.c_int64 * 16)()
         n = _lib.check(_lib.lib().eph_profile_read(self.h, names, avg, cnt, 16))
         return {names[i].decode(): (avg[i], cnt[i]) for i in range(n)}
+
+
+class MSystem:
+    """A system loaded from the reference's text inputs (msystem of the
+    reference: load / accel / integrate), resident on the GPU."""
+
+    CPU_RK12, GPU_RK12 = 0, 1
+
+    def __init__(self, config_path, summation_at_load=_lib.SUMMATION_SERIAL, device=0):
+        _lib.init(device)
+        h = C.c_void_p()
+        _lib.check(_lib.lib().eph_msystem_load(str(config_path).encode(), summation_at_load, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.lib().eph_msystem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return int(_lib.lib().eph_msystem_size(self.h))
+
+    @property
+    def masses(self):
+        m = np.zeros(len(self), MASS_DTYPE)
+        _lib.check(_lib.lib().eph_msystem_masses(self.h, _ptr(m), len(m)))
+        return m
+
+    def models(self):
+        ng, nr = C.c_int64(), C.c_int64()
+        _lib.check(_lib.lib().eph_msystem_models(self.h, None, 0, None, 0, C.byref(ng), C.byref(nr)))
+        gps, rings = np.zeros(ng.value, GP_DTYPE), np.zeros(nr.value, RING_DTYPE)
+        _lib.check(_lib.lib().eph_msystem_models(self.h, _ptr(gps), len(gps), _ptr(rings), len(rings), None, None))
+        return gps, rings
+
+    def t_eph(self):
+        t = (C.c_double * 2)()
+        _lib.check(_lib.lib().eph_msystem_time(self.h, t))
+        return t[0], t[1]
+
+    def accel(self, parallel_option=0):
+        _lib.check(_lib.lib().eph_msystem_accel(self.h, parallel_option))
+
+    def integrate(self, dt, n_step, use_gpu=1):
+        _lib.check(_lib.lib().eph_msystem_integrate(self.h, float(dt), int(n_step), int(use_gpu)))
+
+    def diagnostics(self):
+        n = _lib.check(_lib.lib().eph_msystem_diagnostics(self.h, None, None, None, None, 0))
+        body = (C.c_int64 * max(n, 1))()
+        kind = (C.c_int * max(n, 1))()
+        val = (C.c_double * max(n, 1))()
+        lim = (C.c_double * max(n, 1))()
+        _lib.check(_lib.lib().eph_msystem_diagnostics(self.h, body, kind, val, lim, n))
+        return [(body[i], ("collision", "timestep")[kind[i]], val[i], lim[i]) for i in range(n)]

@@ -132,6 +132,30 @@ int eph_profile_enable(eph_system *sys, int enable);
 int eph_profile_read(eph_system *sys, const char **names, double *avg_ms, int64_t *launches, int cap);
 int eph_profile_reset(eph_system *sys);
 
+/* ---- host system object (text inputs -> resident system) ------------------
+ * The msystem operator interface of the reference around the calls above:
+ *   eph_msystem_load       <- msystem::load(fconfig)      src/modules/mass_loader.cpp:67-294, 324-561
+ *   eph_msystem_accel      <- msystem::accel(int)         src/physics/mass.cpp:216-397
+ *   eph_msystem_integrate  <- msystem::integrate(dt,n,USE_GPU) src/physics/mass.cpp:399-442
+ * USE_GPU must be 1 (GPU_RK12): no CPU back end is provided (EPH_ENOTSUP).
+ * parallel_option -1 selects the single-thread summation order, 0 and 1 the
+ * thread-pool order.  Diagnostics mirror the collision (kind 0: value = the
+ * closest approach, limit = R) and step-size (kind 1: value = |dt|, limit =
+ * recommended dt) reports of mass.cpp:411-439; eph_msystem_diagnostics
+ * returns their number and copies up to cap of them. */
+typedef struct eph_msystem eph_msystem;
+int eph_msystem_load(const char *config_path, int summation_at_load, eph_msystem **out);
+int eph_msystem_destroy(eph_msystem *h);
+int64_t eph_msystem_size(const eph_msystem *h);
+int eph_msystem_masses(const eph_msystem *h, eph_mass *out, int64_t cap);
+int eph_msystem_models(const eph_msystem *h, eph_geopotential *gps, int64_t gcap, eph_ring *rings,
+                       int64_t rcap, int64_t *n_gp, int64_t *n_ring);
+int eph_msystem_time(const eph_msystem *h, double *t_eph);
+int eph_msystem_accel(eph_msystem *h, int parallel_option);
+int eph_msystem_integrate(eph_msystem *h, double dt, int64_t n_step, int use_gpu);
+int eph_msystem_diagnostics(const eph_msystem *h, int64_t *body, int *kind, double *value,
+                            double *limit, int64_t cap);
+
 #ifdef __cplusplus
 }
 #endif
