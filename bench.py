@@ -52,6 +52,33 @@ def cpu_baseline(masses_n, sample_steps):
             "rk12_steps_per_s": sample_steps / el}
 
 
+def batched_leg(steps, warmup, n_clones):
+    """256 clones of the 504-body system (N ~ U(64,504)) in one batch."""
+    from ephemeris_amd.ensemble import Ensemble
+    t0 = time.perf_counter()
+    ens = Ensemble.synthetic(n_clones)
+    t_load = time.perf_counter() - t0
+    ens.integrate(DT, warmup)
+    g = ens.gpu
+    g.profile(True)
+    g.profile_reset()
+    t0 = time.perf_counter()
+    ens.integrate(DT, steps)
+    el = time.perf_counter() - t0
+    prof = g.profile_read()
+    g.profile(False)
+    pairs = ens.pairs_per_eval()
+    main_ms = prof["main_force_pool"][0]
+    achieved = pairs * FLOP_MAIN_PAIR / (main_ms * 1e-3) / 1e12
+    return {"members": ens.n_members, "bodies": int(ens.sys_n.sum()), "pairs_per_eval": pairs,
+            "steps": steps, "pair_interactions_per_s": steps * 25 * pairs / el,
+            "rk12_member_steps_per_s": steps * ens.n_members / el, "ms_per_step": el / steps * 1e3,
+            "load_s": t_load,
+            "roofline_main_force": {"achieved": achieved, "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                    "frac": achieved / FP64_VECTOR_PEAK_TFLOPS, "avg_launch_ms": main_ms},
+            "kernel_ms": {k: round(v[0], 5) for k, v in prof.items()}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,6 +86,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=4)
+    ap.add_argument("--clones", type=int, default=256)
+    ap.add_argument("--no-batch", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported (each system, clone or particle "
@@ -98,6 +128,10 @@ def main():
            "config": {"workload": "full504: 504-body full-model system, RK12 dt=300 s, plain single-system path",
                       "bodies": n, "parallelism": "single GPU"},
            "rk12_steps_per_s": a.steps / el, "roofline": roofline}
+    if not a.no_batch:
+        b = batched_leg(a.batch_steps, 1, a.clones)
+        b["vs_plain_pair_rate"] = b["pair_interactions_per_s"] / value
+        out["batched"] = b
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, a.cpu_steps)
     print(json.dumps(out))

@@ -265,141 +265,203 @@ __device__ void finalize_body(const View &d, const Acc &A, int g, v3 da, v3 dq) 
 // ---------------------------------------------------------------------------
 struct JDeform { double rxh, rxl, ryh, ryl, rzh, rzl, vxh, vxl, vyh, vyl, vzh, vzl, GM, sR2, tdf, pad; };
 
-template <int BS>
-__global__ __launch_bounds__(BS) void k_deform(View d, const int2 *__restrict__ blk, int prepare) {
-    const Acc A{d.F, d.nbp};
-    const int s = blk[blockIdx.x].x, i0 = blk[blockIdx.x].y;
-    const SysMeta sm = d.sys[s];
-    const int n = sm.n, off = sm.off;
-    const int il = i0 + (int)threadIdx.x;
-    const bool valid = il < n;
-    const int gi = off + (valid ? il : 0);
-    __shared__ JDeform tile[BS];
+// Per-body constants of UPDATE_HARMONICS (mass.impl:48-61); the static part
+// s C_static(t) s^T does not depend on the spin, so it is formed once.
+struct Harmonics {
+    double k2r, scale, R2, A;
+    m3 Cw;
+    __device__ void load(const Acc &Ac, int g) {
+        k2r = Ac.ld(F_K2R, g);
+        R2 = Ac.ld(F_R2, g);
+        A = Ac.ld(F_A, g);
+        scale = Ac.ld(F_RAD, g) * R2 / (2 * Ac.ld(F_GM, g));
+        const double exJ2 = Ac.ld(F_EXJ2, g);
+        m3 mc = Ac.ldm(F_CST, g);
+        mc.x.x += exJ2 / 2;
+        mc.y.y += exJ2 / 2;
+        mc.z.z -= exJ2;
+        Cw = toworld(todbl(Ac.lddm(F_S, g)), mc);
+    }
+    // C_potential and GI from the tidal sum and the spin w
+    __device__ void apply(const m3 &Csum, v3 w, m3 &C, m3 &GI) const {
+        const double w2 = dot(w, w);
+        C = adddiag(Csum, w2 * k2r / 3);
+        C = C - outer(k2r * w, w);
+        C = scale * C;
+        C = C + Cw;
+        GI = (-2.0 / 3 * R2) * subdiag(C, A);
+    }
+};
 
-    const ddv3 ri = A.lddv(F_R, gi), vi = A.lddv(F_V, gi);
-    v3 w = A.ldv(F_W, gi);
-    const double sR2 = A.ld(F_SR2, gi), k2 = A.ld(F_K2, gi), td = A.ld(F_TD, gi);
-    const int cls = k2 != 0 ? (td != 0 ? 0 : 1) : 2;
+// UPDATE_ANGULAR_VELOCITY (mass.impl:63-72); returns should_break
+__device__ __forceinline__ bool update_spin(const Acc &A, int g, const m3 &GI, v3 &w, const ddv3 &GL) {
+    if (finite(GL)) {
+        const v3 oldw = w;
+        w = inverse(GI) * todbl(GL);
+        const v3 dwv = w - oldw;
+        return dot(dwv, dwv) < 1e-18 * dot(oldw, oldw);
+    }
+    A.stddv(F_GL, g, mkddv(GI * w));
+    return true;
+}
+
+__device__ __forceinline__ void load_jdeform(const Acc &A, JDeform &T, int gj) {
+    T.rxh = A.ld(F_R + 0, gj); T.rxl = A.ld(F_R + 1, gj);
+    T.ryh = A.ld(F_R + 2, gj); T.ryl = A.ld(F_R + 3, gj);
+    T.rzh = A.ld(F_R + 4, gj); T.rzl = A.ld(F_R + 5, gj);
+    T.vxh = A.ld(F_V + 0, gj); T.vxl = A.ld(F_V + 1, gj);
+    T.vyh = A.ld(F_V + 2, gj); T.vyl = A.ld(F_V + 3, gj);
+    T.vzh = A.ld(F_V + 4, gj); T.vzl = A.ld(F_V + 5, gj);
+    T.GM = A.ld(F_GM, gj); T.sR2 = A.ld(F_SR2, gj); T.tdf = A.ld(F_TDF, gj);
+}
+
+__device__ __forceinline__ void deform_write(const Acc &A, int g, int prepare, const ddv3 &vi, double phi, v3 nacc,
+                                             const m3 &C, const m3 &GI, v3 w) {
+    A.stm(F_CPOT, g, C);
+    A.stm(F_GI, g, GI);
+    A.stv(F_W, g, w);
+    if (prepare) {                                         // PREPARE_RELATIVITY, mass.impl:76-80
+        const v3 beta = todbl(vi) / kC;
+        A.stv(F_BETA, g, beta);
+        A.st(F_BETA2, g, dot(beta, beta));
+        A.st(F_PHIC, g, phi / kC2);
+        A.stv(F_NACCC, g, nacc / kC2);
+    } else {                                               // plain msystem::deform()
+        A.st(F_PHI, g, phi);
+        A.stv(F_NACC, g, nacc);
+    }
+}
+
+// Bodies without delayed tides (k2 == 0, or tide_delay == 0): the partner sum
+// of deform_by does not depend on the spin, so the up-to-four passes of the
+// spin fixed point (mass.cpp:123-129) all re-add the same terms; it is summed
+// once and only UPDATE_HARMONICS / UPDATE_ANGULAR_VELOCITY are iterated.
+// blk[b] = {system, first slot, slot count}; slots map to bodies via d.perm.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_deform_light(View d, const int4 *__restrict__ blk, int prepare) {
+    const Acc A{d.F, d.nbp};
+    const int4 bk = blk[blockIdx.x];
+    const SysMeta sm = d.sys[bk.x];
+    const int n = sm.n, off = sm.off;
+    const bool valid = (int)threadIdx.x < bk.z;
+    const int gi = valid ? d.perm[off + bk.y + threadIdx.x] : off;
+    const int il = gi - off;
+    __shared__ JDeform tile[BS];
+    const ddv3 ri = A.lddv(F_R, gi);
+    const double sR2 = A.ld(F_SR2, gi), k2 = A.ld(F_K2, gi);
+    const bool ndelay = k2 != 0;
     double phi = 0;
     v3 nacc{0, 0, 0};
-    m3 C = mzero();
-    int iters_left = 4;                                    // MAX_ANGULAR_VELOCITY_ITER
+    m3 Csum = mzero();
+    for (int t0 = 0; t0 < n; t0 += BS) {
+        __syncthreads();
+        if (t0 + (int)threadIdx.x < n) load_jdeform(A, tile[threadIdx.x], off + t0 + (int)threadIdx.x);
+        __syncthreads();
+        if (!valid) continue;
+        const int tn = min(BS, n - t0);
+        for (int q = 0; q < tn; ++q) {
+            if (t0 + q == il) continue;
+            const JDeform &T = tile[q];
+            const ddv3 rj{dd{T.rxh, T.rxl}, dd{T.ryh, T.ryl}, dd{T.rzh, T.rzl}};
+            const v3 r = diff(rj, ri);
+            const double rr2 = 1 / smax(dot(r, r), sR2 + T.sR2);
+            const double rr = sqrt(rr2);
+            const double tp_dphi = rr * T.GM;
+            const double tp_dg = rr2 * tp_dphi;
+            phi -= tp_dphi;                                // _NK2 (mass.impl:38-46) / _NDELAY (:25-36)
+            nacc += tp_dg * r;
+            if (ndelay) {
+                const double fmj = -3 * tp_dg;
+                Csum = Csum - (fmj * k2) * subdiag(outer(rr2 * r, r), 1.0 / 3);
+            }
+        }
+    }
+    if (!valid) return;
+    Harmonics H;
+    H.load(A, gi);
+    v3 w = A.ldv(F_W, gi);
+    const ddv3 GL = A.lddv(F_GL, gi);
+    m3 C, GI;
+    int iters = 4;                                         // MAX_ANGULAR_VELOCITY_ITER
+    for (;;) {
+        H.apply(Csum, w, C, GI);
+        if (!prepare || update_spin(A, gi, GI, w, GL) || --iters == 0) break;
+    }
+    deform_write(A, gi, prepare, A.lddv(F_V, gi), phi, nacc, C, GI, w);
+}
+
+// Bodies with constant-time-lag tides (k2 != 0, tide_delay != 0): the tidal
+// phase depends on the spin, so each fixed-point pass re-sums the partners
+// (DAMPED_TIDAL_DEFORMATION_MATRIX, mass.impl:1-23).
+template <int BS>
+__global__ __launch_bounds__(BS) void k_deform_damped(View d, const int4 *__restrict__ blk, int prepare) {
+    const Acc A{d.F, d.nbp};
+    const int4 bk = blk[blockIdx.x];
+    const SysMeta sm = d.sys[bk.x];
+    const int n = sm.n, off = sm.off;
+    const bool valid = (int)threadIdx.x < bk.z;
+    const int gi = valid ? d.perm[off + bk.y + threadIdx.x] : off;
+    const int il = gi - off;
+    __shared__ JDeform tile[BS];
+    const ddv3 ri = A.lddv(F_R, gi), vi = A.lddv(F_V, gi);
+    const double sR2 = A.ld(F_SR2, gi), k2 = A.ld(F_K2, gi), td = A.ld(F_TD, gi);
+    const ddv3 GL = A.lddv(F_GL, gi);
+    Harmonics H;
+    H.load(A, gi);
+    v3 w = A.ldv(F_W, gi);
+    double phi = 0;
+    v3 nacc{0, 0, 0};
+    m3 C = mzero(), GI = mzero();
+    int iters_left = 4;
     bool active = valid;
     for (;;) {
-        if (active) {
-            phi = 0;
-            nacc = v3{0, 0, 0};
-            C = mzero();
-        }
+        m3 Csum = mzero();
+        if (active) { phi = 0; nacc = v3{0, 0, 0}; }
         for (int t0 = 0; t0 < n; t0 += BS) {
             __syncthreads();
-            {
-                const int jl = t0 + (int)threadIdx.x;
-                if (jl < n) {
-                    const int gj = off + jl;
-                    JDeform &T = tile[threadIdx.x];
-                    T.rxh = A.ld(F_R + 0, gj); T.rxl = A.ld(F_R + 1, gj);
-                    T.ryh = A.ld(F_R + 2, gj); T.ryl = A.ld(F_R + 3, gj);
-                    T.rzh = A.ld(F_R + 4, gj); T.rzl = A.ld(F_R + 5, gj);
-                    T.vxh = A.ld(F_V + 0, gj); T.vxl = A.ld(F_V + 1, gj);
-                    T.vyh = A.ld(F_V + 2, gj); T.vyl = A.ld(F_V + 3, gj);
-                    T.vzh = A.ld(F_V + 4, gj); T.vzl = A.ld(F_V + 5, gj);
-                    T.GM = A.ld(F_GM, gj); T.sR2 = A.ld(F_SR2, gj); T.tdf = A.ld(F_TDF, gj);
-                }
-            }
+            if (t0 + (int)threadIdx.x < n) load_jdeform(A, tile[threadIdx.x], off + t0 + (int)threadIdx.x);
             __syncthreads();
             if (!active) continue;
             const int tn = min(BS, n - t0);
             for (int q = 0; q < tn; ++q) {
-                const int jl = t0 + q;
-                if (jl == il) continue;
+                if (t0 + q == il) continue;
                 const JDeform &T = tile[q];
                 const ddv3 rj{dd{T.rxh, T.rxl}, dd{T.ryh, T.ryl}, dd{T.rzh, T.rzl}};
+                const ddv3 vj{dd{T.vxh, T.vxl}, dd{T.vyh, T.vyl}, dd{T.vzh, T.vzl}};
                 const v3 r = diff(rj, ri);
+                const v3 v = diff(vj, vi);
                 const double rr2 = 1 / smax(dot(r, r), sR2 + T.sR2);
                 const double rr = sqrt(rr2);
                 const double tp_dphi = rr * T.GM;
-                if (cls == 2) {                            // _NK2, mass.impl:38-46
-                    const double tp_dg = rr2 * tp_dphi;
-                    phi -= tp_dphi;
-                    nacc += tp_dg * r;
-                } else if (cls == 1) {                     // _NDELAY, mass.impl:25-36
-                    phi -= tp_dphi;
-                    const double tp_dg = rr2 * tp_dphi;
-                    nacc += tp_dg * r;
-                    const double fmj = -3 * tp_dg;
-                    C = C - (fmj * k2) * subdiag(outer(rr2 * r, r), 1.0 / 3);
-                } else {                                   // damped, mass.impl:1-23
-                    const ddv3 vj{dd{T.vxh, T.vxl}, dd{T.vyh, T.vyl}, dd{T.vzh, T.vzl}};
-                    const v3 v = diff(vj, vi);
-                    phi -= tp_dphi;
-                    const double tp_dg = rr2 * tp_dphi;
-                    nacc += tp_dg * r;
-                    const double fmj = -3 * tp_dg;
-                    v3 dw = rr2 * cross(r, cross(w, r) - v);
-                    double dwt = normalize(dw);
-                    dwt *= td * T.tdf;
-                    const double ecwt2 = 1 + 4 * dwt * dwt,
-                                 secwt2 = sqrt(ecwt2),
-                                 x0 = sqrt((1 + secwt2) / (2 * ecwt2)),
-                                 y0 = dwt * sqrt(2 / (ecwt2 * (1 + secwt2))),
-                                 z02 = dwt * dwt * (2 / (ecwt2 + secwt2));
-                    const v3 dr = x0 * r + y0 * cross(dw, r);
-                    m3 t = outer(rr2 * dr, dr) - outer(z02 * dw, dw);
-                    t = adddiag(t, z02 - 1.0 / 3);
-                    C = C - (fmj * k2) * t;
-                }
+                phi -= tp_dphi;
+                const double tp_dg = rr2 * tp_dphi;
+                nacc += tp_dg * r;
+                const double fmj = -3 * tp_dg;
+                v3 dw = rr2 * cross(r, cross(w, r) - v);
+                double dwt = normalize(dw);
+                dwt *= td * T.tdf;
+                const double ecwt2 = 1 + 4 * dwt * dwt,
+                             secwt2 = sqrt(ecwt2),
+                             x0 = sqrt((1 + secwt2) / (2 * ecwt2)),
+                             y0 = dwt * sqrt(2 / (ecwt2 * (1 + secwt2))),
+                             z02 = dwt * dwt * (2 / (ecwt2 + secwt2));
+                const v3 dr = x0 * r + y0 * cross(dw, r);
+                m3 t = outer(rr2 * dr, dr) - outer(z02 * dw, dw);
+                t = adddiag(t, z02 - 1.0 / 3);
+                Csum = Csum - (fmj * k2) * t;
             }
         }
         bool again = false;
         if (active) {
-            // UPDATE_HARMONICS, mass.impl:48-61
-            const double k2r = A.ld(F_K2R, gi), Rr = A.ld(F_RAD, gi), R2 = A.ld(F_R2, gi);
-            const double GM = A.ld(F_GM, gi), exJ2 = A.ld(F_EXJ2, gi), Ai = A.ld(F_A, gi);
-            const double w2 = dot(w, w);
-            C = adddiag(C, w2 * k2r / 3);
-            C = C - outer(k2r * w, w);
-            C = (Rr * R2 / (2 * GM)) * C;
-            const m3 fmis = todbl(A.lddm(F_S, gi));
-            m3 mc = A.ldm(F_CST, gi);
-            mc.x.x += exJ2 / 2;
-            mc.y.y += exJ2 / 2;
-            mc.z.z -= exJ2;
-            C = C + toworld(fmis, mc);
-            const m3 GI = (-2.0 / 3 * R2) * subdiag(C, Ai);
-            // UPDATE_ANGULAR_VELOCITY, mass.impl:63-72 (not part of msystem::deform)
-            const ddv3 GL = A.lddv(F_GL, gi);
-            bool brk;
-            if (!prepare) {
-                brk = true;
-            } else if (finite(GL)) {
-                const v3 oldw = w;
-                w = inverse(GI) * todbl(GL);
-                const v3 dwv = w - oldw;
-                brk = dot(dwv, dwv) < 1e-18 * dot(oldw, oldw);
-            } else {
-                A.stddv(F_GL, gi, mkddv(GI * w));
-                brk = true;
-            }
-            A.stm(F_GI, gi, GI);
+            H.apply(Csum, w, C, GI);
+            const bool brk = !prepare || update_spin(A, gi, GI, w, GL);
             again = !brk && --iters_left > 0;
-            if (!again) active = false;
+            active = again;
         }
         if (!__syncthreads_or(again)) break;
     }
     if (!valid) return;
-    A.stm(F_CPOT, gi, C);
-    A.stv(F_W, gi, w);
-    if (prepare) {                                         // PREPARE_RELATIVITY, mass.impl:76-80
-        const v3 beta = todbl(vi) / kC;
-        A.stv(F_BETA, gi, beta);
-        A.st(F_BETA2, gi, dot(beta, beta));
-        A.st(F_PHIC, gi, phi / kC2);
-        A.stv(F_NACCC, gi, nacc / kC2);
-    } else {                                               // plain msystem::deform()
-        A.st(F_PHI, gi, phi);
-        A.stv(F_NACC, gi, nacc);
-    }
+    deform_write(A, gi, prepare, vi, phi, nacc, C, GI, w);
 }
 
 // ---------------------------------------------------------------------------
@@ -800,8 +862,10 @@ void set_rk_table(const double *tab650, hipStream_t st) {
     (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(c_rk), tab650, sizeof(double) * 650, 0, hipMemcpyHostToDevice, st);
 }
 
-void launch_deform(const View &d, const int2 *blk, int nblk, int prepare, hipStream_t st) {
-    if (nblk) hipLaunchKernelGGL(k_deform<kBS>, dim3(nblk), dim3(kBS), 0, st, d, blk, prepare);
+void launch_deform(const View &d, const int4 *blk_light, int n_light, const int4 *blk_damped, int n_damped,
+                   int prepare, hipStream_t st) {
+    if (n_damped) hipLaunchKernelGGL(k_deform_damped<kBS>, dim3(n_damped), dim3(kBS), 0, st, d, blk_damped, prepare);
+    if (n_light) hipLaunchKernelGGL(k_deform_light<kBS>, dim3(n_light), dim3(kBS), 0, st, d, blk_light, prepare);
 }
 void launch_main_pool(const View &d, const int2 *blk, int nblk, hipStream_t st) {
     if (nblk) hipLaunchKernelGGL(k_main_pool<kBS>, dim3(nblk), dim3(kBS), 0, st, d, blk);

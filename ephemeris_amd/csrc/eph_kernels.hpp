@@ -21,13 +21,15 @@ struct View {
     const int32_t *np_gp;        // per body: geopotential model or -1
     const int32_t *np_ring;      // per body: ring model or -1
     const int32_t *bsys;         // per body: system index
+    const int32_t *perm;         // per system slot: body, bodies with delayed tides first
     double *an;                  // non-point field per (item, partner), 3 doubles
     double *part;                // per (item, round) partial sums, 6 doubles
     const double *t0;            // per system: t_eph (hi+lo) at the start of integrate
 };
 
 void set_rk_table(const double *tab650, hipStream_t st);
-void launch_deform(const View &d, const int2 *blk, int nblk, int prepare, hipStream_t st);
+void launch_deform(const View &d, const int4 *blk_light, int n_light, const int4 *blk_damped, int n_damped,
+                   int prepare, hipStream_t st);
 void launch_main_pool(const View &d, const int2 *blk, int nblk, hipStream_t st);
 void launch_main_serial(const View &d, const int2 *blk, int nblk, hipStream_t st);
 void launch_np(const View &d, int nrows, int max_n, int maxT, const int2 *recv_blk, int nrecv, hipStream_t st);

@@ -126,6 +126,7 @@ struct Scope {
 Batch::~Batch() {
     profiler.collect();
     for (void *p : dev_allocs) (void)hipFree(p);
+    for (void *p : class_allocs) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -227,6 +228,7 @@ View Batch::view(int mode) const {
     v.np_gp = d_np_gp;
     v.np_ring = d_np_ring;
     v.bsys = d_bsys;
+    v.perm = d_perm;
     v.an = pl.d_an;
     v.part = pl.d_part;
     v.t0 = d_t0;
@@ -238,7 +240,7 @@ int Batch::accel(int mode, bool prepare_only) {
     const Plan &pl = plan[mode];
     {
         Scope sc(profiler, 1, stream);
-        launch_deform(v, d_blk_all, n_blk_all, prepare_only ? 0 : 1, stream);
+        launch_deform(v, d_blk_light, n_blk_light, d_blk_damped, n_blk_damped, prepare_only ? 0 : 1, stream);
     }
     if (prepare_only) return EPH_OK;
     if (pl.n_pool) {
@@ -288,7 +290,50 @@ int Batch::integrate(double dt, int64_t n_step, int mode) {
     return EPH_OK;
 }
 
+// Slot permutation per system: bodies with delayed tides (k2 != 0 and
+// tide_delay != 0) first, then the rest, each in index order; the deform
+// kernels take one class per workgroup.  Partner order is unaffected.
+int Batch::build_class_tables(const eph_mass *masses) {
+    std::vector<int32_t> perm(nb);
+    std::vector<int4> light, damped;
+    for (int64_t s = 0; s < n_sys; ++s) {
+        int64_t k = sys_off[s];
+        for (int pass = 0; pass < 2; ++pass)
+            for (int64_t g = sys_off[s]; g < sys_off[s + 1]; ++g) {
+                const bool dmp = masses[g].k2 != 0 && masses[g].tide_delay != 0;
+                if (dmp == (pass == 0)) perm[k++] = (int32_t)g;
+            }
+        int64_t nd = 0;
+        for (int64_t g = sys_off[s]; g < sys_off[s + 1]; ++g) nd += masses[g].k2 != 0 && masses[g].tide_delay != 0;
+        for (int64_t q = 0; q < nd; q += kBS) damped.push_back(make_int4((int)s, (int)q, (int)std::min<int64_t>(kBS, nd - q), 0));
+        for (int64_t q = nd; q < sys_n[s]; q += kBS)
+            light.push_back(make_int4((int)s, (int)q, (int)std::min<int64_t>(kBS, sys_n[s] - q), 0));
+    }
+    HIP_CHECK(hipStreamSynchronize(stream));
+    for (void *p : class_allocs) (void)hipFree(p);
+    class_allocs.clear();
+    auto up = [&](auto *&dst, const auto &src) -> int {
+        dst = nullptr;
+        if (src.empty()) return EPH_OK;
+        HIP_CHECK(hipMalloc(&dst, sizeof(src[0]) * src.size()));
+        class_allocs.push_back(dst);
+        HIP_CHECK(hipMemcpy(dst, src.data(), sizeof(src[0]) * src.size(), hipMemcpyHostToDevice));
+        return EPH_OK;
+    };
+    int rc;
+    if ((rc = up(d_perm, perm))) return rc;
+    if ((rc = up(d_blk_light, light))) return rc;
+    if ((rc = up(d_blk_damped, damped))) return rc;
+    n_blk_light = (int)light.size();
+    n_blk_damped = (int)damped.size();
+    return EPH_OK;
+}
+
 int Batch::upload(const eph_mass *masses) {
+    {
+        int rc = build_class_tables(masses);
+        if (rc) return rc;
+    }
     std::vector<double> buf((size_t)kMassFieldEnd * nbp, 0.0);
     for (int64_t g = 0; g < nb; ++g) {
         const char *m = (const char *)&masses[g];

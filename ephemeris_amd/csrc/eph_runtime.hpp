@@ -59,12 +59,18 @@ struct Batch {
     double *d_t0 = nullptr;
     int2 *d_blk_all = nullptr;
     int n_blk_all = 0;
+    // deform work split by tidal class (see build_class_tables)
+    int32_t *d_perm = nullptr;
+    int4 *d_blk_light = nullptr, *d_blk_damped = nullptr;
+    int n_blk_light = 0, n_blk_damped = 0;
+    std::vector<void *> class_allocs;
     Plan plan[2];
     std::vector<void *> dev_allocs;
     Profiler profiler;
 
     ~Batch();
     int build_plan(int mode);
+    int build_class_tables(const eph_mass *masses);
     View view(int mode) const;
     int accel(int mode, bool deform_only);
     int integrate(double dt, int64_t n_step, int mode);

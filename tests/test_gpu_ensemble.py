@@ -1,0 +1,45 @@
+"""Batched ensembles: every member of a batch matches the same member
+integrated alone (bit for bit) and the oracle on the same text inputs."""
+import numpy as np
+import pytest
+
+from ephemeris_amd import ensemble, generator as gen
+from ephemeris_amd.system import GpuSystems
+from oracle import pyoracle as po
+from tests.util import bits_equal, position
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def members(tmp_path_factory):
+    base = gen.make_full()
+    specs = gen.make_ensemble(6, seed=99, base=base, n_lo=40, n_hi=200)
+    return ensemble.write_members(specs, tmp_path_factory.mktemp("ens"))
+
+
+def test_member_alone_equals_member_in_batch(members):
+    print(f"\n[case] ensemble of {len(members)} members, 4 steps", flush=True)
+    ens = ensemble.Ensemble(members)
+    ens.integrate(300.0, 4)
+    mb, tb = ens.download()
+    for k in (0, 3, 5):
+        alone = ensemble.Ensemble([members[k]])
+        alone.integrate(300.0, 4)
+        ma, ta = alone.download()
+        assert bits_equal(ens.member(mb, k), ma), k
+        assert tb[k].tolist() == ta[0].tolist()
+
+
+def test_members_match_oracle(members):
+    print("\n[case] ensemble members vs oracle, 4 steps", flush=True)
+    ens = ensemble.Ensemble(members)
+    ens.integrate(300.0, 4)
+    mb, _ = ens.download()
+    for k in (1, 4):
+        o = po.OSystem.load(members[k], pool_at_load=False)
+        o.integrate(300.0, 4, pool=True)
+        mine, ref = ens.member(mb, k), o.masses
+        dr = np.max(np.linalg.norm(position(mine) - position(ref), axis=1) / np.linalg.norm(position(ref), axis=1))
+        print(f"member {k} (N={len(ref)}): max rel dr = {dr:.3e}", flush=True)
+        assert dr <= 1e-13
