@@ -201,11 +201,23 @@ def _barycentre(bodies: Sequence[Body]):
         b.v = b.v - vc
 
 
+def _deformation_j2(b: Body, parent: Optional[Body] = None, r_parent: float = 0.0) -> float:
+    """J2 of the rotational (k2r) and parent-tidal (k2) deformation that the
+    loader removes from the input J2 (mass_loader.cpp:540-551)."""
+    R = abs(b.R_km) * 1000.0
+    w2 = float(np.dot(b.w, b.w))
+    j2 = b.k2r * w2 * R ** 3 / (3 * b.GM)
+    if parent is not None and r_parent > 0:
+        j2 += b.k2 * parent.GM * R ** 3 / (2 * b.GM * r_parent ** 3)
+    return j2
+
+
 def _star(rng, tides):
     x, z, w = _spin_frame(rng, np.array([0.0, 0.0, 1.0]), 609.12, 0.13)
     s = Body("Star", "S0", G * M_SUN, 696000.0, inertia=0.0688, J2=2e-7, x_axis=x, z_axis=z, w=w)
     if tides:
         s.k2, s.k2r, s.td = 0.03, 0.027, 0.1
+        s.J2 = 2e-7 + 1.5 * _deformation_j2(s)
     return s
 
 
@@ -226,6 +238,7 @@ def _planets(rng, star, n=8, tides=False):
             b.k2 = rng.uniform(0.1, 0.5)
             b.k2r = b.k2
             b.td = rng.uniform(0.0, 600.0)
+            b.J2 += 1.5 * _deformation_j2(b, star, a)
         b._a, b._mass = a, mass
         out.append(b)
     return out
@@ -258,6 +271,7 @@ def _moons(rng, planet, count, prefix, major_ratio=None, tides=False, rmin=5.0, 
                 m.k2 = rng.uniform(0.02, 0.3)
                 m.k2r = m.k2
                 m.td = rng.uniform(0.0, 600.0)
+                m.J2 += 1.5 * _deformation_j2(m, planet, a * (1 - e))
         out.append(m)
     return out
 

@@ -1022,6 +1022,34 @@ static int load_ring(const char *path, double ref_GM, double ref_R, double dmf, 
     return 1;
 }
 
+/* mass::sanity, src/modules/mass_loader.cpp:18-65 */
+static int o_sanity(const o_mass *m, char *why, size_t cap) {
+    const double eps = 0x1p-53, eps2 = eps * eps, large_finite = 1 / eps2;
+    int ok = 1;
+    why[0] = 0;
+#define O_FAIL(msg) do { ok = 0; strncat(why, msg "; ", cap - strlen(why) - 1); } while (0)
+    o_v3 r = ddv2v(m->r), v = ddv2v(m->v);
+    if (!(vdot(r, r) < large_finite * large_finite) || !(vdot(v, v) < C2_LIGHT)) O_FAIL("position/velocity not finite");
+    o_m3 f = ddm2m(m->s);
+    o_m3 o = madddiag(mneg(mmulm(mtranspose(f), f)), 1);
+    if (!(vdot(o.x, o.x) + vdot(o.y, o.y) + vdot(o.z, o.z) < 64 * eps2) || !(mdet(f) > 0)) O_FAIL("rotation not normalized");
+    if (!(vdot(m->w, m->w) * m->R2 < C2_LIGHT)) O_FAIL("rotation too fast");
+    if (!(eps2 < m->GM0 && m->GM0 < large_finite)) O_FAIL("invalid gravitational parameter");
+    if (!(2 * m->GM0 < C2_LIGHT * fabs(m->R))) O_FAIL("invalid radius");
+    if (!(eps2 < m->inertia && m->inertia <= 1)) O_FAIL("invalid inertia factor");
+    if (!(0 <= m->k2 && m->k2 <= 1.5 && 0 <= m->k2r && m->k2r <= 1.5)) O_FAIL("invalid tidal/rotational deformation factor");
+    if (!(0 <= m->tide_delay && m->tide_delay < large_finite && 0 <= m->tide_delay_factor && m->tide_delay_factor <= large_finite))
+        O_FAIL("invalid tidal delay time/factor");
+    const o_m3 C = m->C_static;
+    double j2 = -C.z.z, c21 = (C.x.z + C.z.x) / 3, c22 = (C.x.x - C.y.y) / 6, s21 = (C.y.z + C.z.y) / 3, s22 = (C.x.y + C.y.x) / 6;
+    if (!(0 <= j2 && j2 + 6 * fabs(c22) < 3 * m->inertia && fabs(c21) < m->inertia && fabs(s21) < m->inertia && fabs(s22) * 2 < m->inertia))
+        O_FAIL("invalid static harmonics");
+    if (m->R < 0 && (m->k2 || m->k2r || j2 || m->dJ2 || c21 || c22 || s21 || s22 || m->gp >= 0 || m->ring >= 0 || m->tide_delay))
+        O_FAIL("soft mass shall not have non-point components");
+#undef O_FAIL
+    return ok;
+}
+
 static o_system *sys_new(void) {
     o_system *s = (o_system *)calloc(1, sizeof(o_system));
     s->nthreads = 1;
@@ -1145,7 +1173,15 @@ o_system *o_load_files(const char *initial, const char *extra, const char *gp_di
         o_m3 ncp = mtoworld(fmis, m->C_static);
         m->C_static = msub(m->C_static, msub(mtolocal(fmis, m->C_potential), m->C_static));
         m->C_potential = ncp;
+        char why[256];
+        if (!o_sanity(m, why, sizeof why)) {
+            char id[9] = {0};
+            memcpy(id, &m->sid, 8);
+            snprintf(g_err, sizeof g_err, "sanity check failed: <%s>: %s", id, why);
+            failed = 1;
+        }
     }
+    if (failed) { o_free(s); return NULL; }
     o_accel(s, 0, pool_at_load);
     return s;
 }
