@@ -79,6 +79,31 @@ def batched_leg(steps, warmup, n_clones):
             "kernel_ms": {k: round(v[0], 5) for k, v in prof.items()}}
 
 
+def particles_leg(n_particles, steps):
+    """Massless particles in the field of the star and 8 planets, dt = 1 day."""
+    import tempfile
+    from ephemeris_amd import generator as gen
+    from ephemeris_amd.particles import ParticleSet
+    f9 = gen.make_field9(gen.make_full())
+    cfg = gen.write_system(f9, tempfile.mkdtemp(prefix="eph_f9_"), "Field9")
+    r, v = gen.make_particles(n_particles, field9=f9)
+    ps = ParticleSet.from_config(cfg, r, v)
+    ps.integrate(86400.0, 1)
+    g = ps.field
+    g.profile(True)
+    g.profile_reset()
+    t0 = time.perf_counter()
+    ps.integrate(86400.0, steps)
+    el = time.perf_counter() - t0
+    prof = g.profile_read()
+    g.profile(False)
+    nf = len(f9.bodies)
+    return {"particles": n_particles, "field_bodies": nf, "steps": steps,
+            "particle_steps_per_s": n_particles * steps / el,
+            "pair_interactions_per_s": n_particles * steps * 25 * nf / el, "ms_per_step": el / steps * 1e3,
+            "kernel_ms": {k: round(v[0], 4) for k, v in prof.items() if v[1]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +114,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=4)
     ap.add_argument("--clones", type=int, default=256)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--particles", type=int, default=1000000)
+    ap.add_argument("--particle-steps", type=int, default=4)
+    ap.add_argument("--no-particles", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported (each system, clone or particle "
@@ -132,6 +160,8 @@ def main():
         b = batched_leg(a.batch_steps, 1, a.clones)
         b["vs_plain_pair_rate"] = b["pair_interactions_per_s"] / value
         out["batched"] = b
+    if not a.no_particles:
+        out["particles"] = particles_leg(a.particles, a.particle_steps)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, a.cpu_steps)
     print(json.dumps(out))

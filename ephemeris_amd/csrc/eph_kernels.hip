@@ -467,12 +467,6 @@ __global__ __launch_bounds__(BS) void k_deform_damped(View d, const int4 *__rest
 // ---------------------------------------------------------------------------
 // main force, thread-pool summation order (mass.cpp:136-157)
 // ---------------------------------------------------------------------------
-struct JMain {
-    double rxh, rxl, ryh, ryl, rzh, rzl, GM, sR2;
-    double phi, beta2, bx, by, bz, nx, ny, nz;
-    double C[9], R2, GLx, GLy, GLz, lum, pad0, pad1;
-};
-
 __device__ __forceinline__ void load_jmain(const Acc &A, JMain &T, int gj) {
     T.rxh = A.ld(F_R + 0, gj); T.rxl = A.ld(F_R + 1, gj);
     T.ryh = A.ld(F_R + 2, gj); T.ryl = A.ld(F_R + 3, gj);
@@ -775,10 +769,7 @@ __global__ void k_np_recv(View d, const int2 *__restrict__ blk, int maxT) {
 // ---------------------------------------------------------------------------
 // RK12 stage: store f[k-1], combine, update state (RungeKutta.impl:113-186)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_stage(View d, int k, double dt, double dt_k, double tadd) {
-    const Acc A{d.F, d.nbp};
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= d.nb) return;
+__device__ void stage_body(const View &d, const Acc &A, int g, int k, double dt, double dt_k, double tadd) {
     ddv3 v = A.lddv(F_V, g);
     if (k == 1) {
         A.stddv(F_X0R, g, A.lddv(F_R, g));
@@ -847,12 +838,303 @@ __global__ __launch_bounds__(256) void k_stage(View d, int k, double dt, double 
     A.st(F_EXJ2, g, A.ld(F_DJ2, g) * t);
 }
 
+__global__ __launch_bounds__(256) void k_stage(View d, int k, double dt, double dt_k, double tadd) {
+    const Acc A{d.F, d.nbp};
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= d.nb) return;
+    stage_body(d, A, g, k, dt, dt_k, tadd);
+}
+
+// ---------------------------------------------------------------------------
+// whole-system-in-one-workgroup path: every body of the system sits in one
+// LDS tile, so deform and main force need no further tile traffic
+// ---------------------------------------------------------------------------
+__device__ void deform_body_tile(const Acc &A, int gi, int il, int n, const JDeform *tile) {
+    const ddv3 ri = A.lddv(F_R, gi), vi = A.lddv(F_V, gi);
+    const double sR2 = A.ld(F_SR2, gi), k2 = A.ld(F_K2, gi), td = A.ld(F_TD, gi);
+    const ddv3 GL = A.lddv(F_GL, gi);
+    Harmonics H;
+    H.load(A, gi);
+    v3 w = A.ldv(F_W, gi);
+    double phi = 0;
+    v3 nacc{0, 0, 0};
+    m3 C, GI, Csum;
+    if (!(k2 != 0 && td != 0)) {                           // spin-independent partner sum
+        Csum = mzero();
+        for (int q = 0; q < n; ++q) {
+            if (q == il) continue;
+            const JDeform &T = tile[q];
+            const ddv3 rj{dd{T.rxh, T.rxl}, dd{T.ryh, T.ryl}, dd{T.rzh, T.rzl}};
+            const v3 r = diff(rj, ri);
+            const double rr2 = 1 / smax(dot(r, r), sR2 + T.sR2);
+            const double rr = sqrt(rr2);
+            const double tp_dphi = rr * T.GM;
+            const double tp_dg = rr2 * tp_dphi;
+            phi -= tp_dphi;
+            nacc += tp_dg * r;
+            if (k2 != 0) {
+                const double fmj = -3 * tp_dg;
+                Csum = Csum - (fmj * k2) * subdiag(outer(rr2 * r, r), 1.0 / 3);
+            }
+        }
+        int iters = 4;
+        for (;;) {
+            H.apply(Csum, w, C, GI);
+            if (update_spin(A, gi, GI, w, GL) || --iters == 0) break;
+        }
+    } else {
+        for (int iters = 4; iters > 0; --iters) {
+            phi = 0;
+            nacc = v3{0, 0, 0};
+            Csum = mzero();
+            for (int q = 0; q < n; ++q) {
+                if (q == il) continue;
+                const JDeform &T = tile[q];
+                const ddv3 rj{dd{T.rxh, T.rxl}, dd{T.ryh, T.ryl}, dd{T.rzh, T.rzl}};
+                const ddv3 vj{dd{T.vxh, T.vxl}, dd{T.vyh, T.vyl}, dd{T.vzh, T.vzl}};
+                const v3 r = diff(rj, ri);
+                const v3 v = diff(vj, vi);
+                const double rr2 = 1 / smax(dot(r, r), sR2 + T.sR2);
+                const double rr = sqrt(rr2);
+                const double tp_dphi = rr * T.GM;
+                phi -= tp_dphi;
+                const double tp_dg = rr2 * tp_dphi;
+                nacc += tp_dg * r;
+                const double fmj = -3 * tp_dg;
+                v3 dw = rr2 * cross(r, cross(w, r) - v);
+                double dwt = normalize(dw);
+                dwt *= td * T.tdf;
+                const double ecwt2 = 1 + 4 * dwt * dwt,
+                             secwt2 = sqrt(ecwt2),
+                             x0 = sqrt((1 + secwt2) / (2 * ecwt2)),
+                             y0 = dwt * sqrt(2 / (ecwt2 * (1 + secwt2))),
+                             z02 = dwt * dwt * (2 / (ecwt2 + secwt2));
+                const v3 dr = x0 * r + y0 * cross(dw, r);
+                m3 t = outer(rr2 * dr, dr) - outer(z02 * dw, dw);
+                t = adddiag(t, z02 - 1.0 / 3);
+                Csum = Csum - (fmj * k2) * t;
+            }
+            H.apply(Csum, w, C, GI);
+            if (update_spin(A, gi, GI, w, GL)) break;
+        }
+    }
+    deform_write(A, gi, 1, vi, phi, nacc, C, GI, w);
+}
+
+__device__ void main_serial_body_tile(const View &d, const Acc &A, int gi, int il, int n, const JMain *tile) {
+    const int off = gi - il;
+    const IMain I = load_imain(A, gi);
+    const int my_gp = d.np_gp[gi], my_ring = d.np_ring[gi];
+    v3 ga{0, 0, 0}, da{0, 0, 0}, dq{0, 0, 0};
+    double mind = A.ld(F_MIND, gi), maxi = A.ld(F_MAXI, gi);
+    for (int q = 0; q < il; ++q) serial_reaction(d, A, off + q, gi, tile[q], I, da);
+    for (int q = 0; q < n; ++q) {
+        if (q == il) continue;
+        const JMain &T = tile[q];
+        const PairGeo p = relativity(I, T, ga);
+        mind = checked_max_into(mind, p.rr);
+        maxi = checked_max_into(maxi, p.tp_dg);
+        const v3 dg = (p.rr3 * p.rr2 * I.R2) * quad_field(I.C, p);
+        da += T.GM * dg;
+        dq += T.GM * cross(p.r, dg);
+        lt_rad(I, T, p, da, dq);
+        if (my_gp >= 0) {
+            const v3 an = nonpoint_field(d, A, gi, 0, my_gp, p.r);
+            da -= T.GM * an;
+            dq -= T.GM * cross(p.r, an);
+        }
+        if (my_ring >= 0) {
+            const v3 an = nonpoint_field(d, A, gi, 1, my_ring, p.r);
+            da -= T.GM * an;
+            dq -= T.GM * cross(p.r, an);
+        }
+    }
+    for (int q = il + 1; q < n; ++q) serial_reaction(d, A, off + q, gi, tile[q], I, da);
+    A.stv(F_GACC, gi, ga);
+    A.st(F_MIND, gi, mind);
+    A.st(F_MAXI, gi, maxi);
+    finalize_body(d, A, gi, da, dq);
+}
+
+// One workgroup integrates one system (N <= kSmallMax bodies, single-thread
+// summation order) for n_steps RK12 steps with no launch per stage.  With
+// `rec`, the partner data (JMain) of every body after each stage's deform is
+// recorded: entry e = 25*step + (k-1) of the launch, workgroup b, body t at
+// rec[(e*nrec_sys + b)*kSmallMax + t].
+__global__ __launch_bounds__(kSmallMax) void k_small(View d, const int *__restrict__ sys_ids, int n_steps,
+                                                     double dt, DtK dtk, int64_t i_step0, JMain *rec, int nrec_sys) {
+    const Acc A{d.F, d.nbp};
+    const int s = sys_ids[blockIdx.x];
+    const SysMeta sm = d.sys[s];
+    const int n = sm.n, off = sm.off, t = (int)threadIdx.x;
+    const bool valid = t < n;
+    __shared__ JDeform tdf[kSmallMax];
+    __shared__ JMain tmn[kSmallMax];
+    for (int step = 0; step < n_steps; ++step) {
+        for (int k = 1; k <= 25; ++k) {
+            const double tadd = dt * (dtk.v[k] + (double)(i_step0 + step));
+            if (valid) stage_body(d, A, off + t, k, dt, dtk.v[k], tadd);
+            __syncthreads();
+            if (valid) load_jdeform(A, tdf[t], off + t);
+            __syncthreads();
+            if (valid) deform_body_tile(A, off + t, t, n, tdf);
+            __syncthreads();
+            if (valid) load_jmain(A, tmn[t], off + t);
+            __syncthreads();
+            if (valid) main_serial_body_tile(d, A, off + t, t, n, tmn);
+            if (rec && valid) {
+                const int64_t e = 25 * (int64_t)step + (k - 1);
+                rec[(e * nrec_sys + blockIdx.x) * kSmallMax + t] = tmn[t];
+            }
+            __syncthreads();
+        }
+    }
+}
+
 __global__ void k_extrema(View d, int invert) {
     const Acc A{d.F, d.nbp};
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= d.nb) return;
     if (invert) A.st(F_MIND, g, 1 / A.ld(F_MIND, g));
     else { A.st(F_MIND, g, 0.0); A.st(F_MAXI, g, 0.0); }
+}
+
+
+// ---------------------------------------------------------------------------
+// massless test particles in the field of a small system (config of the
+// reference: the particle is one more body, after the field bodies, of a
+// system with N <= 48, so the single-thread summation order applies)
+// ---------------------------------------------------------------------------
+// Acceleration of one particle at state (r, v) from the field records T[0..nf)
+// (mass.cpp:311-374 for body index nf of an (nf+1)-body system).  The field
+// bodies' own sums are unaffected by the particle: its terms sit far below
+// their last ulp.  Terms that are exactly zero for a body without spin,
+// deformation or receptance (own tidal torque/force, own Lense-Thirring,
+// radiation received) are skipped: adding +-0 never changes a sum that
+// starts at +0 in round-to-nearest.
+__device__ __forceinline__ v3 particle_accel(const JMain *__restrict__ T, int nf, const ddv3 &r, const ddv3 &v,
+                                             double sR2, double &mind, double &maxi) {
+    double phi = 0;
+    v3 nacc{0, 0, 0};
+    for (int j = 0; j < nf; ++j) {                          // deform_by, _NK2 class
+        const ddv3 rj{dd{T[j].rxh, T[j].rxl}, dd{T[j].ryh, T[j].ryl}, dd{T[j].rzh, T[j].rzl}};
+        const v3 d = diff(rj, r);
+        const double rr2 = 1 / smax(dot(d, d), sR2 + T[j].sR2);
+        const double rr = sqrt(rr2);
+        const double tp_dphi = rr * T[j].GM;
+        const double tp_dg = rr2 * tp_dphi;
+        phi -= tp_dphi;
+        nacc += tp_dg * d;
+    }
+    IMain I;
+    I.r = r;
+    I.sR2 = sR2;
+    I.beta = todbl(v) / kC;
+    I.beta2 = dot(I.beta, I.beta);
+    I.phi = phi / kC2;
+    const v3 naccc = nacc / kC2;
+    v3 da{0, 0, 0}, ga{0, 0, 0};
+    for (int p = 0; p < nf; ++p) {                          // reactions of earlier bodies' quadrupoles
+        const JMain &Tp = T[p];
+        bool quad = false;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) quad |= Tp.C[q] != 0;
+        if (!quad) continue;
+        const ddv3 rp{dd{Tp.rxh, Tp.rxl}, dd{Tp.ryh, Tp.ryl}, dd{Tp.rzh, Tp.rzl}};
+        PairGeo g;
+        g.r = diff(r, rp);
+        g.rr2 = 1 / smax(dot(g.r, g.r), Tp.sR2 + sR2);
+        g.rr = sqrt(g.rr2);
+        g.rr3 = g.rr * g.rr2;
+        const m3 Cp{v3{Tp.C[0], Tp.C[1], Tp.C[2]}, v3{Tp.C[3], Tp.C[4], Tp.C[5]}, v3{Tp.C[6], Tp.C[7], Tp.C[8]}};
+        da -= Tp.GM * ((g.rr3 * g.rr2 * Tp.R2) * quad_field(Cp, g));
+    }
+    for (int j = 0; j < nf; ++j) {                          // own partner loop
+        const JMain &Tj = T[j];
+        const PairGeo g = relativity(I, Tj, ga);
+        mind = checked_max_into(mind, g.rr);
+        maxi = checked_max_into(maxi, g.tp_dg);
+        if (Tj.GLx != 0 || Tj.GLy != 0 || Tj.GLz != 0) {    // LENSE_THIRRING, the partner's spin
+            const double rcr32 = 2 * Tj.GM / kC * g.rr3;
+            const v3 GLj{Tj.GLx, Tj.GLy, Tj.GLz};
+            da += cross(rcr32 * (GLj - (3 * dot(GLj, g.r) * g.rr2) * g.r), g.b);
+        }
+    }
+    return (ga + da) + naccc * kC2;                          // gaccel + daccel + naccel
+}
+
+// Particle fields (stride pv.npp): r (6), v (6), last acceleration (3),
+// min_distance, max_influence.
+enum PField : int { P_R = 0, P_V = 6, P_A = 12, P_MIND = 15, P_MAXI = 16, P_COUNT = 17 };
+
+__global__ __launch_bounds__(kPartBS) void k_particles(PView pv, int n_steps, double dt, DtK dtk, int init) {
+    const int p = blockIdx.x * kPartBS + (int)threadIdx.x;
+    const bool valid = p < pv.np;
+    const int q = valid ? p : 0;
+    double *P = pv.P;
+    const int64_t S = pv.npp;
+    auto ld = [&](int f) { return P[f * S + q]; };
+    ddv3 r{dd{ld(P_R + 0), ld(P_R + 1)}, dd{ld(P_R + 2), ld(P_R + 3)}, dd{ld(P_R + 4), ld(P_R + 5)}};
+    ddv3 v{dd{ld(P_V + 0), ld(P_V + 1)}, dd{ld(P_V + 2), ld(P_V + 3)}, dd{ld(P_V + 4), ld(P_V + 5)}};
+    double mind = ld(P_MIND), maxi = ld(P_MAXI);
+    if (init) {
+        const v3 a = particle_accel(pv.rec, pv.nf, r, v, pv.sR2, mind, maxi);
+        if (valid) {
+            P[(P_A + 0) * S + q] = a.x; P[(P_A + 1) * S + q] = a.y; P[(P_A + 2) * S + q] = a.z;
+            P[P_MIND * S + q] = mind; P[P_MAXI * S + q] = maxi;
+        }
+        return;
+    }
+    // stage derivatives a_j (j = 0..24) of this step, per particle
+    __shared__ double fa[25 * 3 * kPartBS];
+    auto FA = [&](int j, int c) -> double & { return fa[(j * 3 + c) * kPartBS + threadIdx.x]; };
+    v3 a{ld(P_A + 0), ld(P_A + 1), ld(P_A + 2)};
+    for (int step = 0; step < n_steps; ++step) {
+        const ddv3 r0 = r, v0 = v;
+        FA(0, 0) = a.x; FA(0, 1) = a.y; FA(0, 2) = a.z;
+        for (int k = 1; k <= 25; ++k) {
+            // x_k = x0 + dt * sum_j a_kj f_j (RungeKutta.impl:148-170); the
+            // velocity of stage j is v0 + dt * (sum_l a_jl a_l), recomputed
+            // from the stored accelerations in the same order
+            ddv3 cv = mkddv(v3{0, 0, 0});
+            v3 ca{0, 0, 0};
+            const int base = k * (k - 1) / 2;
+            for (int j = 0; j < k; ++j) {
+                const double2 c = c_rk[base + j];
+                if (c.x == 0) continue;
+                const dd ckj{c.x, c.y};
+                const double fc = c.x + c.y;
+                ddv3 vj = v0;
+                if (j > 0) {
+                    v3 caj{0, 0, 0};
+                    const int bj = j * (j - 1) / 2;
+                    for (int l = 0; l < j; ++l) {
+                        const double2 cl = c_rk[bj + l];
+                        if (cl.x == 0) continue;
+                        caj += (cl.x + cl.y) * v3{FA(l, 0), FA(l, 1), FA(l, 2)};
+                    }
+                    vj = add(v0, mkddv(dt * caj));
+                }
+                cv = add(cv, scale(ckj, vj));
+                ca += fc * v3{FA(j, 0), FA(j, 1), FA(j, 2)};
+            }
+            r = add(r0, scale(mkdd(dt), cv));
+            v = add(v0, mkddv(dt * ca));
+            const JMain *T = pv.rec + ((int64_t)(25 * step + (k - 1)) * pv.nrec_sys) * kSmallMax;
+            a = particle_accel(T, pv.nf, r, v, pv.sR2, mind, maxi);
+            if (k < 25) { FA(k, 0) = a.x; FA(k, 1) = a.y; FA(k, 2) = a.z; }
+        }
+    }
+    if (!valid) return;
+    const double out[P_COUNT] = {r.x.hi, r.x.lo, r.y.hi, r.y.lo, r.z.hi, r.z.lo,
+                                 v.x.hi, v.x.lo, v.y.hi, v.y.lo, v.z.hi, v.z.lo, a.x, a.y, a.z, mind, maxi};
+#pragma unroll
+    for (int f = 0; f < P_COUNT; ++f) P[f * S + q] = out[f];
+}
+
+void launch_particles(const PView &pv, int n_steps, double dt, const DtK &dtk, int init, hipStream_t st) {
+    if (pv.np) hipLaunchKernelGGL(k_particles, dim3((pv.np + kPartBS - 1) / kPartBS), dim3(kPartBS), 0, st, pv,
+                                  n_steps, dt, dtk, init);
 }
 
 // ---------------------------------------------------------------------------
@@ -882,6 +1164,10 @@ void launch_np(const View &d, int nrows, int max_n, int maxT, const int2 *recv_b
 }
 void launch_stage(const View &d, int k, double dt, double dt_k, double tadd, hipStream_t st) {
     hipLaunchKernelGGL(k_stage, dim3((d.nb + 255) / 256), dim3(256), 0, st, d, k, dt, dt_k, tadd);
+}
+void launch_small(const View &d, const int *sys_ids, int nsys, int n_steps, double dt, const DtK &dtk,
+                  int64_t i_step0, JMain *rec, hipStream_t st) {
+    if (nsys) hipLaunchKernelGGL(k_small, dim3(nsys), dim3(kSmallMax), 0, st, d, sys_ids, n_steps, dt, dtk, i_step0, rec, nsys);
 }
 void launch_extrema(const View &d, int invert, hipStream_t st) {
     hipLaunchKernelGGL(k_extrema, dim3((d.nb + 255) / 256), dim3(256), 0, st, d, invert);

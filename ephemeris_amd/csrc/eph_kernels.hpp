@@ -9,6 +9,31 @@
 namespace eph {
 
 constexpr int kBS = 64;          // bodies per workgroup in the j-loop kernels (one wave64)
+constexpr int kSmallMax = 64;    // largest system integrated by one workgroup
+
+// partner data of one body as the main force reads it (also the record
+// format of a field system for test particles); 32 doubles
+struct JMain {
+    double rxh, rxl, ryh, ryl, rzh, rzl, GM, sR2;
+    double phi, beta2, bx, by, bz, nx, ny, nz;
+    double C[9], R2, GLx, GLy, GLz, lum, pad0, pad1;
+};
+
+// sum of the non-zero tableau weights of each stage (dt_k), index 1..25
+struct DtK { double v[26]; };
+
+constexpr int kPartBS = 256;     // particles per workgroup (stage storage in LDS: 600 B each)
+
+// test particles in the recorded field of one small system
+struct PView {
+    double *P;                   // particle fields, field-major, stride npp
+    int64_t npp;
+    int np;
+    const JMain *rec;            // field records of the launch (see k_small)
+    int nrec_sys;                // systems interleaved in rec (the field is the first)
+    int nf;                      // field bodies
+    double sR2;                  // soft radius^2 of the particles (0 for R > 0)
+};
 
 struct View {
     double *F;                   // field-major state, F[f * nbp + g]
@@ -35,5 +60,8 @@ void launch_main_serial(const View &d, const int2 *blk, int nblk, hipStream_t st
 void launch_np(const View &d, int nrows, int max_n, int maxT, const int2 *recv_blk, int nrecv, hipStream_t st);
 void launch_stage(const View &d, int k, double dt, double dt_k, double tadd, hipStream_t st);
 void launch_extrema(const View &d, int invert, hipStream_t st);
+void launch_particles(const PView &pv, int n_steps, double dt, const DtK &dtk, int init, hipStream_t st);
+void launch_small(const View &d, const int *sys_ids, int nsys, int n_steps, double dt, const DtK &dtk,
+                  int64_t i_step0, JMain *rec, hipStream_t st);
 
 }  // namespace eph

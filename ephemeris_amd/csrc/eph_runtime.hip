@@ -71,7 +71,8 @@ constexpr int kMassFieldEnd = F_X0R;   // fields [0, F_X0R) carry eph_mass data
 
 // ---- profiling --------------------------------------------------------------
 const char *const kKernelNames[kNumKernelClasses] = {
-    "stage_update", "deform_spin", "main_force_pool", "main_force_serial", "nonpoint"};
+    "stage_update", "deform_spin", "main_force_pool", "main_force_serial", "nonpoint", "small_system_steps",
+    "particle_steps"};
 
 void Profiler::begin(int cls, hipStream_t st) {
     if (!enabled) return;
@@ -258,8 +259,58 @@ int Batch::accel(int mode, bool prepare_only) {
     return EPH_OK;
 }
 
+static DtK stage_dtk() {                                    // RungeKutta.impl:148-152
+    DtK d;
+    d.v[0] = 0;
+    for (int k = 1; k <= 25; ++k) {
+        double dt_k = 0;
+        for (int j = 0; j < k; ++j) {
+            const double *c = kRK[k * (k - 1) / 2 + j];
+            if (c[0] != 0) dt_k += c[0] + c[1];
+        }
+        d.v[k] = dt_k;
+    }
+    return d;
+}
+
+bool Batch::all_small() const {
+    return plan[0].n_pool == 0 && plan[0].n_serial > 0 &&
+           *std::max_element(sys_n.begin(), sys_n.end()) <= kSmallMax;
+}
+
+// One launch runs many steps for every system (one workgroup per system).
+// With `rec`, the field records of each chunk of rec_steps steps are written
+// and after_chunk(ctx, first_step, steps) runs before the next chunk.
+int Batch::integrate_small(double dt, int64_t n_step, JMain *rec, int64_t rec_steps,
+                           void (*after_chunk)(void *, int64_t, int64_t), void *ctx) {
+    if (n_step == 0) return EPH_OK;
+    if (n_step < 0) { n_step = -n_step; dt = -dt; }
+    std::vector<double> t0(n_sys);
+    for (int64_t s = 0; s < n_sys; ++s) t0[s] = todbl(t_eph[s]);
+    HIP_CHECK(hipMemcpyAsync(d_t0, t0.data(), sizeof(double) * n_sys, hipMemcpyHostToDevice, stream));
+    const View v = view(0);
+    const DtK dtk = stage_dtk();
+    launch_extrema(v, 0, stream);
+    const int64_t chunk = rec ? rec_steps : std::min<int64_t>(n_step, 64);
+    for (int64_t s0 = 0; s0 < n_step; s0 += chunk) {
+        const int64_t ns = std::min(chunk, n_step - s0);
+        {
+            Scope sc(profiler, 5, stream);
+            launch_small(v, d_small_ids, (int)small_ids.size(), (int)ns, dt, dtk, s0, rec, stream);
+        }
+        if (after_chunk) after_chunk(ctx, s0, ns);
+    }
+    launch_extrema(v, 1, stream);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(stream));
+    for (int64_t s = 0; s < n_sys; ++s) t_eph[s] = host_add(t_eph[s], mkdd(dt * (double)n_step));
+    return EPH_OK;
+}
+
 int Batch::integrate(double dt, int64_t n_step, int mode) {
     if (n_step == 0) return EPH_OK;
+    if ((mode == 1 || plan[0].n_pool == 0) && all_small() && !force_generic)
+        return integrate_small(dt, n_step, nullptr, 0, nullptr, nullptr);
     if (n_step < 0) { n_step = -n_step; dt = -dt; }
     std::vector<double> t0(n_sys);
     for (int64_t s = 0; s < n_sys; ++s) t0[s] = todbl(t_eph[s]);
@@ -432,10 +483,122 @@ int create_batch(int64_t n_sys, const int64_t *sys_n, const eph_mass *masses,
     b->n_blk_all = (int)blk_all.size();
     if ((rc = b->build_plan(0))) return bail(rc);
     if ((rc = b->build_plan(1))) return bail(rc);
+    for (int64_t s = 0; s < n_sys; ++s) b->small_ids.push_back((int)s);
+    if ((rc = dev_upload(*b, b->d_small_ids, b->small_ids))) return bail(rc);
     set_rk_table(&kRK[0][0], b->stream);
     if ((rc = b->upload(masses))) return bail(rc);
     if (hipStreamSynchronize(b->stream) != hipSuccess) return bail(fail(EPH_EDEVICE, "stream sync failed"));
     *out = b;
+    return EPH_OK;
+}
+
+
+// ---- test particles -------------------------------------------------------------
+Particles::~Particles() {
+    if (P) (void)hipFree(P);
+    if (rec) (void)hipFree(rec);
+}
+
+static constexpr int64_t kRecChunk = 128;      // steps of field records per chunk
+
+int create_particles(Batch *field, int64_t n, const double *r, const double *v, double GM, double R, Particles **out) {
+    if (!field || !out || n < 0 || (n && (!r || !v))) return fail(EPH_EINVAL, "eph_particles_create: bad argument");
+    if (field->n_sys != 1 || !field->all_small())
+        return fail(EPH_EINVAL, "the particle field must be one system of at most 48 bodies (single-thread order)");
+    auto *ps = new Particles();
+    ps->field = field;
+    ps->np = n;
+    ps->npp = (n + 255) / 256 * 256;
+    if (ps->npp == 0) ps->npp = 256;
+    ps->GM = GM;
+    ps->R = R;
+    std::vector<double> host((size_t)P_COUNT_HOST * ps->npp, 0.0);
+    for (int64_t i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) {
+            host[(size_t)(2 * c) * ps->npp + i] = r[3 * i + c];
+            host[(size_t)(6 + 2 * c) * ps->npp + i] = v[3 * i + c];
+        }
+    auto bail = [&](int code) { delete ps; return code; };
+    if (hipMalloc(&ps->P, sizeof(double) * host.size()) != hipSuccess) return bail(fail(EPH_ENOMEM, "particle state allocation failed"));
+    if (hipMemcpy(ps->P, host.data(), sizeof(double) * host.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(EPH_EDEVICE, "particle upload failed"));
+    ps->rec_steps = kRecChunk;
+    const size_t rec_bytes = sizeof(JMain) * kSmallMax * (size_t)(25 * kRecChunk);
+    if (hipMalloc(&ps->rec, rec_bytes) != hipSuccess) return bail(fail(EPH_ENOMEM, "field record allocation failed"));
+    // initial acceleration of each particle from the field's current state
+    std::vector<JMain> t0(kSmallMax);
+    {
+        std::vector<eph_mass> fm(field->nb);
+        int rc = field->download(fm.data());
+        if (rc) return bail(rc);
+        for (int64_t j = 0; j < field->nb; ++j) {
+            const eph_mass &m = fm[j];
+            JMain &T = t0[j];
+            memset(&T, 0, sizeof T);
+            T.rxh = m.r.x.hi; T.rxl = m.r.x.lo; T.ryh = m.r.y.hi; T.ryl = m.r.y.lo; T.rzh = m.r.z.hi; T.rzl = m.r.z.lo;
+            T.GM = m.GM; T.sR2 = m.sR2;
+            T.phi = m.phi / kC2;              // placeholder, replaced below by the device values
+            T.beta2 = m.beta2;
+            T.bx = m.beta.x; T.by = m.beta.y; T.bz = m.beta.z;
+            T.R2 = m.R2;
+            T.GLx = m.GL.x.hi + m.GL.x.lo; T.GLy = m.GL.y.hi + m.GL.y.lo; T.GLz = m.GL.z.hi + m.GL.z.lo;
+            T.lum = m.lum;
+            const double *C = &m.C_potential.x.x;
+            for (int q = 0; q < 9; ++q) T.C[q] = C[q];
+        }
+        // phi/c^2 and naccel/c^2 exactly as the last force evaluation left them
+        std::vector<double> pc(field->nb), nc(3 * field->nb);
+        for (int64_t j = 0; j < field->nb; ++j) {
+            if (hipMemcpy(&pc[j], field->F + (size_t)F_PHIC * field->nbp + j, 8, hipMemcpyDeviceToHost) != hipSuccess)
+                return bail(fail(EPH_EDEVICE, "field read failed"));
+            for (int c = 0; c < 3; ++c)
+                if (hipMemcpy(&nc[3 * j + c], field->F + (size_t)(F_NACCC + c) * field->nbp + j, 8, hipMemcpyDeviceToHost) != hipSuccess)
+                    return bail(fail(EPH_EDEVICE, "field read failed"));
+            t0[j].phi = pc[j];
+            t0[j].nx = nc[3 * j]; t0[j].ny = nc[3 * j + 1]; t0[j].nz = nc[3 * j + 2];
+        }
+    }
+    if (hipMemcpy(ps->rec, t0.data(), sizeof(JMain) * kSmallMax, hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(EPH_EDEVICE, "record upload failed"));
+    PView pv{ps->P, ps->npp, (int)ps->np, ps->rec, 1, (int)field->nb, R < 0 ? R * R : 0.0};
+    launch_particles(pv, 0, 0.0, DtK{}, 1, field->stream);
+    if (hipStreamSynchronize(field->stream) != hipSuccess) return bail(fail(EPH_EDEVICE, "particle init failed"));
+    *out = ps;
+    return EPH_OK;
+}
+
+namespace {
+struct PCtx { Particles *ps; double dt; };
+void particles_after_chunk(void *vctx, int64_t first, int64_t steps) {
+    (void)first;
+    PCtx *c = (PCtx *)vctx;
+    Particles *ps = c->ps;
+    PView pv{ps->P, ps->npp, (int)ps->np, ps->rec, 1, (int)ps->field->nb, ps->R < 0 ? ps->R * ps->R : 0.0};
+    Scope sc(ps->field->profiler, 6, ps->field->stream);
+    launch_particles(pv, (int)steps, c->dt, stage_dtk(), 0, ps->field->stream);
+}
+}  // namespace
+
+int integrate_particles(Particles *ps, double dt, int64_t n_step) {
+    if (n_step < 0) return fail(EPH_EINVAL, "particles integrate forward only");
+    PCtx ctx{ps, dt};
+    return ps->field->integrate_small(dt, n_step, ps->rec, ps->rec_steps, particles_after_chunk, &ctx);
+}
+
+int download_particles(Particles *ps, double *r, double *v, double *extrema) {
+    std::vector<double> host((size_t)P_COUNT_HOST * ps->npp);
+    HIP_CHECK(hipStreamSynchronize(ps->field->stream));
+    HIP_CHECK(hipMemcpy(host.data(), ps->P, sizeof(double) * host.size(), hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < ps->np; ++i) {
+        for (int c = 0; c < 6; ++c) {
+            if (r) r[6 * i + c] = host[(size_t)c * ps->npp + i];
+            if (v) v[6 * i + c] = host[(size_t)(6 + c) * ps->npp + i];
+        }
+        if (extrema) {
+            extrema[2 * i] = host[(size_t)15 * ps->npp + i];
+            extrema[2 * i + 1] = host[(size_t)16 * ps->npp + i];
+        }
+    }
     return EPH_OK;
 }
 
@@ -535,6 +698,36 @@ int eph_integrate_ex(eph_system *sys, double dt, int64_t n_step, int summation) 
 
 int eph_integrate(eph_system *sys, double dt, int64_t n_step) {
     return eph_integrate_ex(sys, dt, n_step, EPH_SUMMATION_POOL);
+}
+
+int eph_particles_create(eph_system *field, int64_t n, const double *r, const double *v, double GM, double R,
+                         eph_particles **out) {
+    eph::Particles *ps = nullptr;
+    int rc = eph::create_particles(reinterpret_cast<Batch *>(field), n, r, v, GM, R, &ps);
+    if (rc) return rc;
+    *out = reinterpret_cast<eph_particles *>(ps);
+    return EPH_OK;
+}
+
+int eph_particles_integrate(eph_particles *ps, double dt, int64_t n_step) {
+    if (!ps) return fail(EPH_EINVAL, "null particles");
+    if (!(dt == dt) || dt == 0) return fail(EPH_EINVAL, "dt must be finite and non-zero");
+    return eph::integrate_particles(reinterpret_cast<eph::Particles *>(ps), dt, n_step);
+}
+
+int eph_particles_download(eph_particles *ps, double *r, double *v, double *extrema) {
+    if (!ps) return fail(EPH_EINVAL, "null particles");
+    return eph::download_particles(reinterpret_cast<eph::Particles *>(ps), r, v, extrema);
+}
+
+int64_t eph_particles_count(const eph_particles *ps) {
+    return ps ? reinterpret_cast<const eph::Particles *>(ps)->np : -1;
+}
+
+int eph_particles_destroy(eph_particles *ps) {
+    if (!ps) return fail(EPH_EINVAL, "null particles");
+    delete reinterpret_cast<eph::Particles *>(ps);
+    return EPH_OK;
 }
 
 int eph_profile_enable(eph_system *sys, int enable) {

@@ -11,7 +11,8 @@ namespace eph {
 
 int fail(int code, const std::string &msg);
 
-constexpr int kNumKernelClasses = 5;
+constexpr int kNumKernelClasses = 7;
+constexpr int P_COUNT_HOST = 17;     // particle fields (PField)
 extern const char *const kKernelNames[kNumKernelClasses];
 
 // Per-kernel-class device time from HIP events recorded on the batch stream.
@@ -67,6 +68,7 @@ struct Batch {
     Plan plan[2];
     std::vector<void *> dev_allocs;
     Profiler profiler;
+    bool force_generic = false;  // debugging aid: never take the persistent small path
 
     ~Batch();
     int build_plan(int mode);
@@ -74,9 +76,32 @@ struct Batch {
     View view(int mode) const;
     int accel(int mode, bool deform_only);
     int integrate(double dt, int64_t n_step, int mode);
+    // persistent path: all systems serial and N <= kSmallMax; optional record
+    int integrate_small(double dt, int64_t n_step, JMain *rec, int64_t rec_steps,
+                        void (*after_chunk)(void *, int64_t, int64_t), void *ctx);
+    bool all_small() const;
+    std::vector<int> small_ids;
+    int *d_small_ids = nullptr;
     int upload(const eph_mass *masses);
     int download(eph_mass *masses);
 };
+
+// Massless test particles advanced in the field of one small resident system.
+struct Particles {
+    Batch *field = nullptr;      // not owned; must outlive the particles
+    int field_sys = 0;           // system index of the field in its batch
+    int64_t np = 0, npp = 0;
+    double GM = 0, R = 0;
+    double *P = nullptr;
+    JMain *rec = nullptr;
+    int64_t rec_steps = 0;       // steps recorded per chunk
+    int *d_sys_ids = nullptr;
+    ~Particles();
+};
+
+int create_particles(Batch *field, int64_t n, const double *r, const double *v, double GM, double R, Particles **out);
+int integrate_particles(Particles *ps, double dt, int64_t n_step);
+int download_particles(Particles *ps, double *r, double *v, double *extrema);
 
 int create_batch(int64_t n_sys, const int64_t *sys_n, const eph_mass *masses,
                  const eph_geopotential *gps, int64_t n_gp, const eph_ring *rings, int64_t n_ring,

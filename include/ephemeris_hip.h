@@ -124,6 +124,26 @@ int eph_integrate(eph_system *sys, double dt, int64_t n_step);
 /* As eph_integrate but with an explicit summation order for every accel. */
 int eph_integrate_ex(eph_system *sys, double dt, int64_t n_step, int summation);
 
+/* ---- massless test particles ------------------------------------------------
+ * n particles, each integrated as one more body (appended after the field
+ * bodies) of the `field` system -- which must be a batch of one system of at
+ * most 48 bodies, so the reference's single-thread summation order applies.
+ * A particle exerts no force (its terms are below the field bodies' last
+ * ulp); GM and R are its gravitational parameter and radius.  r and v are
+ * n x 3 FP64 (double-double with lo = 0).  eph_particles_integrate advances
+ * the field system and the particles together; the field must outlive the
+ * particle set.  Downloads: r, v as n x 6 {x.hi,x.lo,y.hi,...}; extrema as
+ * n x {min_distance (running max of 1/r), max_influence}.
+ * No counterpart in the reference's interface: the reference would carry
+ * each particle as an ordinary body (mass_loader.cpp:324-471). */
+typedef struct eph_particles eph_particles;
+int eph_particles_create(eph_system *field, int64_t n, const double *r, const double *v, double GM, double R,
+                         eph_particles **out);
+int eph_particles_integrate(eph_particles *ps, double dt, int64_t n_step);
+int eph_particles_download(eph_particles *ps, double *r, double *v, double *extrema);
+int64_t eph_particles_count(const eph_particles *ps);
+int eph_particles_destroy(eph_particles *ps);
+
 /* ---- measurement ---------------------------------------------------------- */
 /* Average device time (ms) per launch of each kernel class, measured with
  * HIP events on the library's stream since the last reset; names[] gets

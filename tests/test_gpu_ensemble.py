@@ -10,6 +10,9 @@ from tests.util import bits_equal, position
 
 pytestmark = pytest.mark.gpu
 
+# model indices are rebased inside a batch; every other field must match
+STATE_FIELDS = [f for f in __import__("ephemeris_amd").MASS_DTYPE.names if f not in ("gp", "ring")]
+
 
 @pytest.fixture(scope="module")
 def members(tmp_path_factory):
@@ -27,7 +30,8 @@ def test_member_alone_equals_member_in_batch(members):
         alone = ensemble.Ensemble([members[k]])
         alone.integrate(300.0, 4)
         ma, ta = alone.download()
-        assert bits_equal(ens.member(mb, k), ma), k
+        bad = [f for f in STATE_FIELDS if not bits_equal(ens.member(mb, k)[f], ma[f])]
+        assert not bad, (k, bad)
         assert tb[k].tolist() == ta[0].tolist()
 
 
