@@ -502,7 +502,7 @@ __device__ __forceinline__ IMain load_imain(const Acc &A, int gi) {
     return I;
 }
 
-struct PairGeo { v3 r, b; double rr2, rr, rr3, tp_dg; };
+struct PairGeo { v3 r, b, G; double rr2, rr, rr3, tp_dg; };
 
 // RELATIVITY (mass.impl:82-97)
 __device__ __forceinline__ PairGeo relativity(const IMain &I, const JMain &T, v3 &gaccel) {
@@ -521,8 +521,8 @@ __device__ __forceinline__ PairGeo relativity(const IMain &I, const JMain &T, v3
     const double delta1 = 4 * I.phi + T.phi + I.beta2 + 2 * T.beta2
                         - 4 * dot(I.beta, bj) + (dot(p.r, nj) - 3 * tp_rbjrr2) / 2;
     p.b = bj - I.beta;
-    gaccel += ((7 * tp_dphi / 2) * nj + (p.tp_dg * delta1) * p.r)
-              + (p.tp_dg * (rbj - dot(p.r, p.b) * 4)) * p.b;
+    p.G = ((7 * tp_dphi / 2) * nj + (p.tp_dg * delta1) * p.r) + (p.tp_dg * (rbj - dot(p.r, p.b) * 4)) * p.b;
+    gaccel += p.G;
     return p;
 }
 
@@ -585,6 +585,193 @@ __global__ __launch_bounds__(BS) void k_main_pool(View d, const int2 *__restrict
     if (sm.nitem == 0) {                                   // FINALIZE_RELATIVITY
         A.st(F_PHI, gi, I.phi * kC2);
         A.stv(F_NACC, gi, A.ldv(F_NACCC, gi) * kC2);
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// latency path: one wavefront per body.  Each lane evaluates one partner of
+// a 64-partner tile and leaves that pair's terms in LDS; then one lane per
+// accumulator adds them in partner order, so every sum is formed exactly as
+// in the in-order loop above while a 504-body system occupies 504 waves.
+// ---------------------------------------------------------------------------
+constexpr int kWpbTerms = 26;
+
+// main force (thread-pool order, mass.cpp:147-155): accumulators
+// 0-2 gaccel, 3-5 daccel, 6-8 dtorque, 9 min_distance, 10 max_influence
+__global__ __launch_bounds__(64) void k_main_wpb(View d, const int2 *__restrict__ work) {
+    const Acc A{d.F, d.nbp};
+    const int s = work[blockIdx.x].x, il = work[blockIdx.x].y;
+    const SysMeta sm = d.sys[s];
+    const int n = sm.n, off = sm.off, gi = off + il;
+    const int lane = (int)threadIdx.x;
+    __shared__ double term[kWpbTerms][64];
+    const IMain I = load_imain(A, gi);
+    double acc = 0;
+    if (lane == 9) acc = A.ld(F_MIND, gi);
+    if (lane == 10) acc = A.ld(F_MAXI, gi);
+    for (int t0 = 0; t0 < n; t0 += 64) {
+        const int jl = t0 + lane;
+        if (jl < n && jl != il) {
+            JMain T;
+            load_jmain(A, T, off + jl);
+            v3 ga{0, 0, 0};
+            const PairGeo p = relativity(I, T, ga);
+            const v3 dg = (T.GM * p.rr3 * p.rr2 * I.R2) * quad_field(I.C, p);
+            const v3 rxdg = cross(p.r, dg);
+            const m3 Cj{v3{T.C[0], T.C[1], T.C[2]}, v3{T.C[3], T.C[4], T.C[5]}, v3{T.C[6], T.C[7], T.C[8]}};
+            const v3 gj = (T.GM * p.rr3 * p.rr2 * T.R2) * quad_field(Cj, p);
+            const double rcr32 = 2 * T.GM / kC * p.rr3;
+            const v3 GLb = cross(I.GL, p.b);
+            const v3 lt1 = rcr32 * (GLb - (3 * dot(GLb, p.r) * p.rr2) * p.r);
+            const v3 lt2 = rcr32 * cross(I.GL, cross(p.r, p.b));
+            const v3 GLj{T.GLx, T.GLy, T.GLz};
+            const v3 lt3 = cross(rcr32 * (GLj - (3 * dot(GLj, p.r) * p.rr2) * p.r), p.b);
+            const v3 rad = (T.lum * I.rR2G4c * p.rr3 / I.GM) * p.r;
+            const double t[kWpbTerms] = {p.G.x, p.G.y, p.G.z, dg.x, dg.y, dg.z, gj.x, gj.y, gj.z,
+                                         lt1.x, lt1.y, lt1.z, lt3.x, lt3.y, lt3.z, rad.x, rad.y, rad.z,
+                                         rxdg.x, rxdg.y, rxdg.z, lt2.x, lt2.y, lt2.z, p.rr, p.tp_dg};
+#pragma unroll
+            for (int q = 0; q < kWpbTerms; ++q) term[q][lane] = t[q];
+        }
+        __syncthreads();
+        const int tn = min(64, n - t0);
+        if (lane < 3) {
+            for (int q = 0; q < tn; ++q)
+                if (t0 + q != il) acc += term[lane][q];
+        } else if (lane < 6) {
+            const int c = lane - 3;
+            for (int q = 0; q < tn; ++q) {
+                if (t0 + q == il) continue;
+                acc += term[3 + c][q];      // own quadrupole
+                acc += term[6 + c][q];      // partner's quadrupole
+                acc -= term[9 + c][q];      // Lense-Thirring, own spin
+                acc += term[12 + c][q];     // Lense-Thirring, partner's spin
+                acc += term[15 + c][q];     // radiation
+            }
+        } else if (lane < 9) {
+            const int c = lane - 6;
+            for (int q = 0; q < tn; ++q) {
+                if (t0 + q == il) continue;
+                acc += term[18 + c][q];
+                acc -= term[21 + c][q];
+            }
+        } else if (lane < 11) {
+            for (int q = 0; q < tn; ++q)
+                if (t0 + q != il) acc = checked_max_into(acc, term[24 + (lane - 9)][q]);
+        }
+        __syncthreads();
+    }
+    if (lane < 3) A.st(F_GACC + lane, gi, acc);
+    else if (lane < 6) A.st(F_DACC + lane - 3, gi, acc);
+    else if (lane < 9) A.st(F_DTQ + lane - 6, gi, acc);
+    else if (lane == 9) A.st(F_MIND, gi, acc);
+    else if (lane == 10) A.st(F_MAXI, gi, acc);
+    if (sm.nitem == 0 && lane == 11) {                     // FINALIZE_RELATIVITY
+        A.st(F_PHI, gi, I.phi * kC2);
+        A.stv(F_NACC, gi, A.ldv(F_NACCC, gi) * kC2);
+    }
+}
+
+// deform + spin fixed point + PREPARE_RELATIVITY, one wavefront per body.
+// Accumulators: lane 0 phi, 1-3 naccel, 4-12 the tidal matrix.
+__global__ __launch_bounds__(64) void k_deform_wpb(View d, const int2 *__restrict__ work, int prepare) {
+    const Acc A{d.F, d.nbp};
+    const int s = work[blockIdx.x].x, il = work[blockIdx.x].y;
+    const SysMeta sm = d.sys[s];
+    const int n = sm.n, off = sm.off, gi = off + il;
+    const int lane = (int)threadIdx.x;
+    __shared__ double term[13][64];
+    __shared__ double sums[13];
+    __shared__ int again_s;
+    const ddv3 ri = A.lddv(F_R, gi), vi = A.lddv(F_V, gi);
+    const double sR2 = A.ld(F_SR2, gi), k2 = A.ld(F_K2, gi), td = A.ld(F_TD, gi);
+    const bool damped = k2 != 0 && td != 0, ndelay = k2 != 0 && td == 0;
+    const int nacc_chains = (k2 != 0) ? 13 : 4;
+    v3 w = A.ldv(F_W, gi);
+    Harmonics H;
+    m3 C, GI;
+    int iters_left = 4;
+    for (;;) {
+        double acc = 0;
+        for (int t0 = 0; t0 < n; t0 += 64) {
+            const int jl = t0 + lane;
+            if (jl < n && jl != il) {
+                const int gj = off + jl;
+                const v3 r = diff(A.lddv(F_R, gj), ri);
+                const double rr2 = 1 / smax(dot(r, r), sR2 + A.ld(F_SR2, gj));
+                const double rr = sqrt(rr2);
+                const double tp_dphi = rr * A.ld(F_GM, gj);
+                const double tp_dg = rr2 * tp_dphi;
+                const v3 na = tp_dg * r;
+                term[0][lane] = tp_dphi;
+                term[1][lane] = na.x; term[2][lane] = na.y; term[3][lane] = na.z;
+                if (ndelay) {
+                    const double fmj = -3 * tp_dg;
+                    const m3 t = (fmj * k2) * subdiag(outer(rr2 * r, r), 1.0 / 3);
+                    const double *tp = &t.x.x;
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) term[4 + q][lane] = tp[q];
+                } else if (damped) {
+                    const v3 v = diff(A.lddv(F_V, gj), vi);
+                    const double fmj = -3 * tp_dg;
+                    v3 dw = rr2 * cross(r, cross(w, r) - v);
+                    double dwt = normalize(dw);
+                    dwt *= td * A.ld(F_TDF, gj);
+                    const double ecwt2 = 1 + 4 * dwt * dwt,
+                                 secwt2 = sqrt(ecwt2),
+                                 x0 = sqrt((1 + secwt2) / (2 * ecwt2)),
+                                 y0 = dwt * sqrt(2 / (ecwt2 * (1 + secwt2))),
+                                 z02 = dwt * dwt * (2 / (ecwt2 + secwt2));
+                    const v3 dr = x0 * r + y0 * cross(dw, r);
+                    m3 t = outer(rr2 * dr, dr) - outer(z02 * dw, dw);
+                    t = (fmj * k2) * adddiag(t, z02 - 1.0 / 3);
+                    const double *tp = &t.x.x;
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) term[4 + q][lane] = tp[q];
+                }
+            }
+            __syncthreads();
+            const int tn = min(64, n - t0);
+            if (lane == 0) {
+                for (int q = 0; q < tn; ++q)
+                    if (t0 + q != il) acc -= term[0][q];
+            } else if (lane < 4) {
+                for (int q = 0; q < tn; ++q)
+                    if (t0 + q != il) acc += term[lane][q];
+            } else if (lane < nacc_chains) {
+                for (int q = 0; q < tn; ++q)
+                    if (t0 + q != il) acc -= term[lane][q];
+            }
+            __syncthreads();
+        }
+        if (lane < 13) sums[lane] = acc;
+        __syncthreads();
+        if (lane == 0) {
+            const m3 Csum{v3{sums[4], sums[5], sums[6]}, v3{sums[7], sums[8], sums[9]}, v3{sums[10], sums[11], sums[12]}};
+            if (iters_left == 4) H.load(A, gi);
+            const ddv3 GL = A.lddv(F_GL, gi);
+            bool brk;
+            if (damped) {
+                H.apply(Csum, w, C, GI);
+                brk = !prepare || update_spin(A, gi, GI, w, GL);
+                brk = brk || --iters_left == 0;
+            } else {
+                int iters = 4;
+                for (;;) {
+                    H.apply(Csum, w, C, GI);
+                    if (!prepare || update_spin(A, gi, GI, w, GL) || --iters == 0) break;
+                }
+                brk = true;
+            }
+            again_s = brk ? 0 : 1;
+            if (brk) deform_write(A, gi, prepare, vi, sums[0], v3{sums[1], sums[2], sums[3]}, C, GI, w);
+            else A.stv(F_W, gi, w);
+        }
+        __syncthreads();
+        if (!again_s) break;
+        w = A.ldv(F_W, gi);
+        __syncthreads();
     }
 }
 
@@ -1151,6 +1338,12 @@ void launch_deform(const View &d, const int4 *blk_light, int n_light, const int4
 }
 void launch_main_pool(const View &d, const int2 *blk, int nblk, hipStream_t st) {
     if (nblk) hipLaunchKernelGGL(k_main_pool<kBS>, dim3(nblk), dim3(kBS), 0, st, d, blk);
+}
+void launch_main_wpb(const View &d, const int2 *work, int nwork, hipStream_t st) {
+    if (nwork) hipLaunchKernelGGL(k_main_wpb, dim3(nwork), dim3(64), 0, st, d, work);
+}
+void launch_deform_wpb(const View &d, const int2 *work, int nwork, int prepare, hipStream_t st) {
+    if (nwork) hipLaunchKernelGGL(k_deform_wpb, dim3(nwork), dim3(64), 0, st, d, work, prepare);
 }
 void launch_main_serial(const View &d, const int2 *blk, int nblk, hipStream_t st) {
     if (nblk) hipLaunchKernelGGL(k_main_serial<kBS>, dim3(nblk), dim3(kBS), 0, st, d, blk);

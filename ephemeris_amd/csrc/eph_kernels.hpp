@@ -57,6 +57,9 @@ void launch_deform(const View &d, const int4 *blk_light, int n_light, const int4
                    int prepare, hipStream_t st);
 void launch_main_pool(const View &d, const int2 *blk, int nblk, hipStream_t st);
 void launch_main_serial(const View &d, const int2 *blk, int nblk, hipStream_t st);
+// latency path, one wavefront per body; work = {system, body index in system}
+void launch_main_wpb(const View &d, const int2 *work, int nwork, hipStream_t st);
+void launch_deform_wpb(const View &d, const int2 *work, int nwork, int prepare, hipStream_t st);
 void launch_np(const View &d, int nrows, int max_n, int maxT, const int2 *recv_blk, int nrecv, hipStream_t st);
 void launch_stage(const View &d, int k, double dt, double dt_k, double tadd, hipStream_t st);
 void launch_extrema(const View &d, int invert, hipStream_t st);

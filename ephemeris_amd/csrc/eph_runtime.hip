@@ -149,7 +149,7 @@ int Batch::build_plan(int mode) {
     Plan &pl = plan[mode];
     std::vector<SysMeta> meta(n_sys);
     std::vector<NPItem> items;
-    std::vector<int2> blk_pool, blk_serial, blk_recv;
+    std::vector<int2> blk_pool, blk_serial, blk_recv, work_pool;
     int64_t an_rows = 0;
     pl.maxT = 1;
     pl.max_n = 1;
@@ -195,6 +195,7 @@ int Batch::build_plan(int mode) {
         items.insert(items.end(), sys_items.begin(), sys_items.end());
         append_blocks(blk_pool, (int)s, (int)n);
         if (nnp) append_blocks(blk_recv, (int)s, (int)n);
+        for (int64_t i = 0; i < n; ++i) work_pool.push_back(make_int2((int)s, (int)i));
     }
     int rc;
     if ((rc = dev_upload(*this, pl.d_sys, meta))) return rc;
@@ -202,6 +203,8 @@ int Batch::build_plan(int mode) {
     if ((rc = dev_upload(*this, pl.d_blk_pool, blk_pool))) return rc;
     if ((rc = dev_upload(*this, pl.d_blk_serial, blk_serial))) return rc;
     if ((rc = dev_upload(*this, pl.d_blk_recv, blk_recv))) return rc;
+    if ((rc = dev_upload(*this, pl.d_work_pool, work_pool))) return rc;
+    pl.n_work_pool = (int)work_pool.size();
     pl.n_pool = (int)blk_pool.size();
     pl.n_serial = (int)blk_serial.size();
     pl.n_recv = (int)blk_recv.size();
@@ -236,17 +239,30 @@ View Batch::view(int mode) const {
     return v;
 }
 
+// The wave-per-body kernels win while the batch is too small to fill the
+// GPU with one thread per body (fewer than ~16k bodies in thread-pool order).
+static constexpr int kWpbAutoBodies = 16384;
+
+bool Batch::use_wpb(int mode) const {
+    if (variant == 1) return false;
+    if (variant == 2) return true;
+    return plan[mode].n_work_pool > 0 && plan[mode].n_work_pool <= kWpbAutoBodies;
+}
+
 int Batch::accel(int mode, bool prepare_only) {
     const View v = view(mode);
     const Plan &pl = plan[mode];
+    const bool wpb = use_wpb(mode);
     {
         Scope sc(profiler, 1, stream);
-        launch_deform(v, d_blk_light, n_blk_light, d_blk_damped, n_blk_damped, prepare_only ? 0 : 1, stream);
+        if (wpb) launch_deform_wpb(v, d_work_all, n_work_all, prepare_only ? 0 : 1, stream);
+        else launch_deform(v, d_blk_light, n_blk_light, d_blk_damped, n_blk_damped, prepare_only ? 0 : 1, stream);
     }
     if (prepare_only) return EPH_OK;
     if (pl.n_pool) {
         Scope sc(profiler, 2, stream);
-        launch_main_pool(v, pl.d_blk_pool, pl.n_pool, stream);
+        if (wpb) launch_main_wpb(v, pl.d_work_pool, pl.n_work_pool, stream);
+        else launch_main_pool(v, pl.d_blk_pool, pl.n_pool, stream);
     }
     if (pl.n_serial) {
         Scope sc(profiler, 3, stream);
@@ -484,6 +500,13 @@ int create_batch(int64_t n_sys, const int64_t *sys_n, const eph_mass *masses,
     if ((rc = b->build_plan(0))) return bail(rc);
     if ((rc = b->build_plan(1))) return bail(rc);
     for (int64_t s = 0; s < n_sys; ++s) b->small_ids.push_back((int)s);
+    {
+        std::vector<int2> work_all;
+        for (int64_t s = 0; s < n_sys; ++s)
+            for (int64_t i = 0; i < sys_n[s]; ++i) work_all.push_back(make_int2((int)s, (int)i));
+        if ((rc = dev_upload(*b, b->d_work_all, work_all))) return bail(rc);
+        b->n_work_all = (int)work_all.size();
+    }
     if ((rc = dev_upload(*b, b->d_small_ids, b->small_ids))) return bail(rc);
     set_rk_table(&kRK[0][0], b->stream);
     if ((rc = b->upload(masses))) return bail(rc);
@@ -727,6 +750,14 @@ int64_t eph_particles_count(const eph_particles *ps) {
 int eph_particles_destroy(eph_particles *ps) {
     if (!ps) return fail(EPH_EINVAL, "null particles");
     delete reinterpret_cast<eph::Particles *>(ps);
+    return EPH_OK;
+}
+
+int eph_system_set_variant(eph_system *sys, int variant, int allow_persistent) {
+    if (!sys) return fail(EPH_EINVAL, "null system");
+    if (variant < 0 || variant > 2) return fail(EPH_EINVAL, "variant must be 0 (automatic), 1 or 2");
+    B(sys)->variant = variant;
+    B(sys)->force_generic = !allow_persistent;
     return EPH_OK;
 }
 

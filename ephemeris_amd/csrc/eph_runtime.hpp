@@ -39,6 +39,8 @@ struct Plan {
     int nrows = 0, maxT = 1, max_n = 1;
     int64_t an_rows = 0;
     double *d_an = nullptr, *d_part = nullptr;
+    int2 *d_work_pool = nullptr;     // (system, body) of every body of a pool-order system
+    int n_work_pool = 0;
 };
 
 struct Batch {
@@ -68,7 +70,13 @@ struct Batch {
     Plan plan[2];
     std::vector<void *> dev_allocs;
     Profiler profiler;
-    bool force_generic = false;  // debugging aid: never take the persistent small path
+    bool force_generic = false;  // never take the persistent small path
+    // kernel variant of the j-loop passes: 0 = automatic, 1 = one thread per
+    // body, 2 = one wavefront per body
+    int variant = 0;
+    int2 *d_work_all = nullptr;
+    int n_work_all = 0;
+    bool use_wpb(int mode) const;
 
     ~Batch();
     int build_plan(int mode);

@@ -73,6 +73,10 @@ class GpuSystems:
     def integrate(self, dt, n_step, summation=_lib.SUMMATION_POOL):
         _lib.check(_lib.lib().eph_integrate_ex(self.h, float(dt), int(n_step), summation))
 
+    def set_variant(self, variant=0, allow_persistent=True):
+        """0 automatic, 1 one thread per body, 2 one wavefront per body."""
+        _lib.check(_lib.lib().eph_system_set_variant(self.h, int(variant), int(allow_persistent)))
+
     def profile(self, enable=True):
         _lib.check(_lib.lib().eph_profile_enable(self.h, int(enable)))
 

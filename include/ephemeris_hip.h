@@ -124,6 +124,15 @@ int eph_integrate(eph_system *sys, double dt, int64_t n_step);
 /* As eph_integrate but with an explicit summation order for every accel. */
 int eph_integrate_ex(eph_system *sys, double dt, int64_t n_step, int summation);
 
+/* ---- tuning ------------------------------------------------------------------
+ * Kernel variant of the partner-loop passes: 0 automatic, 1 one thread per
+ * body (throughput; large batches), 2 one wavefront per body (latency; few
+ * bodies).  allow_persistent = 0 keeps systems of <= 64 bodies off the
+ * one-workgroup-per-system path.  All choices give bit-identical results.
+ * No counterpart in the reference (its only knob is CUDA_CORES,
+ * src/integrators/RungeKutta.cu:13-18, which changes results). */
+int eph_system_set_variant(eph_system *sys, int variant, int allow_persistent);
+
 /* ---- massless test particles ------------------------------------------------
  * n particles, each integrated as one more body (appended after the field
  * bodies) of the `field` system -- which must be a batch of one system of at

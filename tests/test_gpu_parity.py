@@ -113,3 +113,34 @@ def test_integrate_is_deterministic_and_chunk_consistent():
     ma, _ = a.download()
     mb, _ = b.download()
     assert bits_equal(ma, mb)
+
+
+@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("name,tag", [("full504", "pool"), ("small25", "serial")])
+def test_kernel_variants_bitwise(variant, name, tag):
+    """Per-thread and per-wavefront kernels reproduce the reference bits."""
+    print(f"\n[case] variant {variant} accel {name} {tag}", flush=True)
+    d = load_npz(GOLDEN / f"accel_{name}_{tag}.npz")
+    g = _gpu(d)
+    g.set_variant(variant, allow_persistent=False)
+    g.accel(_lib.SUMMATION_POOL if tag == "pool" else _lib.SUMMATION_SERIAL)
+    m, _ = g.download()
+    bad = [f for f in ACCEL_OUTPUTS if not bits_equal(m[f], d["masses_out"][f])]
+    assert not bad, bad
+
+
+@pytest.mark.parametrize("variant,persistent", [(1, False), (2, False), (0, True)])
+def test_integrate_variants_agree(variant, persistent):
+    print(f"\n[case] integrate full504 and small25 12 steps, variant {variant}", flush=True)
+    for name, tag in [("full504", "pool"), ("small25", "serial")]:
+        d = load_npz(GOLDEN / f"accel_{name}_{tag}.npz")
+        ref = _gpu(d)
+        ref.set_variant(1, allow_persistent=False)
+        ref.integrate(300.0, 12)
+        g = _gpu(d)
+        g.set_variant(variant, allow_persistent=persistent)
+        g.integrate(300.0, 12)
+        a, _ = g.download()
+        b, _ = ref.download()
+        bad = [f for f in a.dtype.names if not bits_equal(a[f], b[f])]
+        assert not bad, (name, bad)
