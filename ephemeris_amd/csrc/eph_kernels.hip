@@ -1254,8 +1254,18 @@ __device__ __forceinline__ v3 particle_accel(const JMain *__restrict__ T, int nf
 // min_distance, max_influence.
 enum PField : int { P_R = 0, P_V = 6, P_A = 12, P_MIND = 15, P_MAXI = 16, P_COUNT = 17 };
 
-__global__ __launch_bounds__(kPartBS) void k_particles(PView pv, int n_steps, double dt, DtK dtk, int init) {
+// n_steps > 0: advance; n_steps < 0: compute the first acceleration from
+// the records; 0: leave the workgroup's particles alone.
+__global__ __launch_bounds__(kPartBS) void k_particles(PView pv, int n_steps, double dt, DtK dtk,
+                                                       const PBlock *__restrict__ blocks) {
     const int p = blockIdx.x * kPartBS + (int)threadIdx.x;
+    int64_t entry0 = 0;
+    if (blocks) {                                           // per-workgroup job offset and length
+        entry0 = blocks[blockIdx.x].rec_entry0;
+        n_steps = blocks[blockIdx.x].n_steps;
+    }
+    if (n_steps == 0) return;
+    const bool init = n_steps < 0;
     const bool valid = p < pv.np;
     const int q = valid ? p : 0;
     double *P = pv.P;
@@ -1265,7 +1275,7 @@ __global__ __launch_bounds__(kPartBS) void k_particles(PView pv, int n_steps, do
     ddv3 v{dd{ld(P_V + 0), ld(P_V + 1)}, dd{ld(P_V + 2), ld(P_V + 3)}, dd{ld(P_V + 4), ld(P_V + 5)}};
     double mind = ld(P_MIND), maxi = ld(P_MAXI);
     if (init) {
-        const v3 a = particle_accel(pv.rec, pv.nf, r, v, pv.sR2, mind, maxi);
+        const v3 a = particle_accel(pv.rec + entry0 * pv.nrec_sys * kSmallMax, pv.nf, r, v, pv.sR2, mind, maxi);
         if (valid) {
             P[(P_A + 0) * S + q] = a.x; P[(P_A + 1) * S + q] = a.y; P[(P_A + 2) * S + q] = a.z;
             P[P_MIND * S + q] = mind; P[P_MAXI * S + q] = maxi;
@@ -1307,7 +1317,7 @@ __global__ __launch_bounds__(kPartBS) void k_particles(PView pv, int n_steps, do
             }
             r = add(r0, scale(mkdd(dt), cv));
             v = add(v0, mkddv(dt * ca));
-            const JMain *T = pv.rec + ((int64_t)(25 * step + (k - 1)) * pv.nrec_sys) * kSmallMax;
+            const JMain *T = pv.rec + ((entry0 + 25 * (int64_t)step + (k - 1)) * pv.nrec_sys) * kSmallMax;
             a = particle_accel(T, pv.nf, r, v, pv.sR2, mind, maxi);
             if (k < 25) { FA(k, 0) = a.x; FA(k, 1) = a.y; FA(k, 2) = a.z; }
         }
@@ -1319,9 +1329,9 @@ __global__ __launch_bounds__(kPartBS) void k_particles(PView pv, int n_steps, do
     for (int f = 0; f < P_COUNT; ++f) P[f * S + q] = out[f];
 }
 
-void launch_particles(const PView &pv, int n_steps, double dt, const DtK &dtk, int init, hipStream_t st) {
+void launch_particles(const PView &pv, int n_steps, double dt, const DtK &dtk, hipStream_t st, const PBlock *blocks) {
     if (pv.np) hipLaunchKernelGGL(k_particles, dim3((pv.np + kPartBS - 1) / kPartBS), dim3(kPartBS), 0, st, pv,
-                                  n_steps, dt, dtk, init);
+                                  n_steps, dt, dtk, blocks);
 }
 
 // ---------------------------------------------------------------------------

@@ -63,7 +63,12 @@ void launch_deform_wpb(const View &d, const int2 *work, int nwork, int prepare, 
 void launch_np(const View &d, int nrows, int max_n, int maxT, const int2 *recv_blk, int nrecv, hipStream_t st);
 void launch_stage(const View &d, int k, double dt, double dt_k, double tadd, hipStream_t st);
 void launch_extrema(const View &d, int invert, hipStream_t st);
-void launch_particles(const PView &pv, int n_steps, double dt, const DtK &dtk, int init, hipStream_t st);
+// per-workgroup schedule of a particle launch: first record entry (25 per
+// step, counted from the field's epoch) and number of steps
+struct PBlock { int64_t rec_entry0; int32_t n_steps, pad; };
+// n_steps < 0 computes the first acceleration from record entry 0
+void launch_particles(const PView &pv, int n_steps, double dt, const DtK &dtk, hipStream_t st,
+                      const PBlock *blocks = nullptr);
 void launch_small(const View &d, const int *sys_ids, int nsys, int n_steps, double dt, const DtK &dtk,
                   int64_t i_step0, JMain *rec, hipStream_t st);
 

@@ -275,7 +275,7 @@ int Batch::accel(int mode, bool prepare_only) {
     return EPH_OK;
 }
 
-static DtK stage_dtk() {                                    // RungeKutta.impl:148-152
+DtK stage_dtk() {                                    // RungeKutta.impl:148-152
     DtK d;
     d.v[0] = 0;
     for (int k = 1; k <= 25; ++k) {
@@ -397,6 +397,9 @@ int Batch::build_class_tables(const eph_mass *masses) {
 }
 
 int Batch::upload(const eph_mass *masses) {
+    host_k2.resize(nb);
+    host_td.resize(nb);
+    for (int64_t g = 0; g < nb; ++g) { host_k2[g] = masses[g].k2; host_td[g] = masses[g].tide_delay; }
     {
         int rc = build_class_tables(masses);
         if (rc) return rc;
@@ -516,6 +519,71 @@ int create_batch(int64_t n_sys, const int64_t *sys_n, const eph_mass *masses,
 }
 
 
+// A new batch holding systems `keep` of `old` (state copied device to device,
+// never through the host) followed by n_new systems given on the host.
+int regroup_batch(Batch *old, const std::vector<int> &keep, int64_t n_new, const int64_t *new_sys_n,
+                  const eph_mass *new_masses, const eph_geopotential *new_gps, int64_t n_new_gp,
+                  const eph_ring *new_rings, int64_t n_new_ring, const double *new_t_eph, Batch **out) {
+    std::vector<int64_t> sys_n;
+    std::vector<double> teph;
+    std::vector<eph_geopotential> gps;
+    std::vector<eph_ring> rings;
+    std::vector<eph_mass> stubs;
+    for (int k : keep) {
+        const int64_t o = old->sys_off[k], n = old->sys_n[k];
+        sys_n.push_back(n);
+        teph.push_back(old->t_eph[k].hi);
+        teph.push_back(old->t_eph[k].lo);
+        for (int64_t g = o; g < o + n; ++g) {
+            eph_mass m;
+            memset(&m, 0, sizeof m);
+            m.k2 = old->host_k2[g];
+            m.tide_delay = old->host_td[g];
+            m.sid = old->sid[g];
+            m.gp = -1;
+            m.ring = -1;
+            if (old->host_gp_global[g] >= 0) { m.gp = (int64_t)gps.size(); gps.push_back(old->gps[old->host_gp_global[g]]); }
+            if (old->host_ring_global[g] >= 0) { m.ring = (int64_t)rings.size(); rings.push_back(old->rings[old->host_ring_global[g]]); }
+            stubs.push_back(m);
+        }
+    }
+    const int64_t gp_base = (int64_t)gps.size(), ring_base = (int64_t)rings.size();
+    int64_t nn = 0;
+    for (int64_t s = 0; s < n_new; ++s) {
+        sys_n.push_back(new_sys_n[s]);
+        teph.push_back(new_t_eph ? new_t_eph[2 * s] : 0.0);
+        teph.push_back(new_t_eph ? new_t_eph[2 * s + 1] : 0.0);
+        nn += new_sys_n[s];
+    }
+    for (int64_t i = 0; i < n_new_gp; ++i) gps.push_back(new_gps[i]);
+    for (int64_t i = 0; i < n_new_ring; ++i) rings.push_back(new_rings[i]);
+    for (int64_t g = 0; g < nn; ++g) {
+        eph_mass m = new_masses[g];
+        if (m.gp >= 0) m.gp += gp_base;
+        if (m.ring >= 0) m.ring += ring_base;
+        stubs.push_back(m);
+    }
+    if (sys_n.empty()) { *out = nullptr; return EPH_OK; }
+    Batch *b = nullptr;
+    int rc = create_batch((int64_t)sys_n.size(), sys_n.data(), stubs.data(), gps.data(), (int64_t)gps.size(),
+                          rings.data(), (int64_t)rings.size(), teph.data(), &b);
+    if (rc) return rc;
+    int64_t dst = 0;
+    for (int k : keep) {
+        const int64_t n = old->sys_n[k];
+        if (hipMemcpy2DAsync(b->F + dst, sizeof(double) * b->nbp, old->F + old->sys_off[k], sizeof(double) * old->nbp,
+                             sizeof(double) * n, kMassFieldEnd, hipMemcpyDeviceToDevice, b->stream) != hipSuccess) {
+            delete b;
+            return fail(EPH_EDEVICE, "device-to-device regroup copy failed");
+        }
+        dst += n;
+    }
+    if (hipStreamSynchronize(b->stream) != hipSuccess) { delete b; return fail(EPH_EDEVICE, "regroup sync failed"); }
+    *out = b;
+    return EPH_OK;
+}
+
+
 // ---- test particles -------------------------------------------------------------
 Particles::~Particles() {
     if (P) (void)hipFree(P);
@@ -584,7 +652,7 @@ int create_particles(Batch *field, int64_t n, const double *r, const double *v, 
     if (hipMemcpy(ps->rec, t0.data(), sizeof(JMain) * kSmallMax, hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail(EPH_EDEVICE, "record upload failed"));
     PView pv{ps->P, ps->npp, (int)ps->np, ps->rec, 1, (int)field->nb, R < 0 ? R * R : 0.0};
-    launch_particles(pv, 0, 0.0, DtK{}, 1, field->stream);
+    launch_particles(pv, -1, 0.0, DtK{}, field->stream);
     if (hipStreamSynchronize(field->stream) != hipSuccess) return bail(fail(EPH_EDEVICE, "particle init failed"));
     *out = ps;
     return EPH_OK;
@@ -598,7 +666,7 @@ void particles_after_chunk(void *vctx, int64_t first, int64_t steps) {
     Particles *ps = c->ps;
     PView pv{ps->P, ps->npp, (int)ps->np, ps->rec, 1, (int)ps->field->nb, ps->R < 0 ? ps->R * ps->R : 0.0};
     Scope sc(ps->field->profiler, 6, ps->field->stream);
-    launch_particles(pv, (int)steps, c->dt, stage_dtk(), 0, ps->field->stream);
+    launch_particles(pv, (int)steps, c->dt, stage_dtk(), ps->field->stream);
 }
 }  // namespace
 

@@ -53,6 +53,7 @@ struct Batch {
     std::vector<dd> t_eph;
     std::vector<uint64_t> sid;
     std::vector<int32_t> host_gp, host_ring;
+    std::vector<double> host_k2, host_td;       // tidal class of each body
     std::vector<int64_t> host_gp_global, host_ring_global;
 
     double *F = nullptr;
@@ -107,6 +108,10 @@ struct Particles {
     ~Particles();
 };
 
+DtK stage_dtk();
+int regroup_batch(Batch *old, const std::vector<int> &keep, int64_t n_new, const int64_t *new_sys_n,
+                  const eph_mass *new_masses, const eph_geopotential *new_gps, int64_t n_new_gp,
+                  const eph_ring *new_rings, int64_t n_new_ring, const double *new_t_eph, Batch **out);
 int create_particles(Batch *field, int64_t n, const double *r, const double *v, double GM, double R, Particles **out);
 int integrate_particles(Particles *ps, double dt, int64_t n_step);
 int download_particles(Particles *ps, double *r, double *v, double *extrema);
