@@ -40,6 +40,19 @@ SIGNATURES = {
     "eph_particles_download": (I, [P, P, P, P]),
     "eph_particles_count": (I64, [P]),
     "eph_particles_destroy": (I, [P]),
+    "eph_server_create": (I, [P, P, PP]),
+    "eph_server_submit_ensemble": (I, [P, I64, P, P, P, I64, P, I64, P, I64, C.POINTER(C.c_int64)]),
+    "eph_server_submit_particles": (I, [P, I64, P, P, I64, C.POINTER(C.c_int64)]),
+    "eph_server_tick": (I, [P, C.POINTER(C.c_int64)]),
+    "eph_server_start": (I, [P]),
+    "eph_server_wait": (I, [P, D]),
+    "eph_server_stop": (I, [P]),
+    "eph_server_job_info": (I, [P, I64, P]),
+    "eph_server_result_ensemble": (I, [P, I64, P, P]),
+    "eph_server_result_particles": (I, [P, I64, P, P, P]),
+    "eph_server_release": (I, [P, I64]),
+    "eph_server_stats_read": (I, [P, P]),
+    "eph_server_destroy": (I, [P]),
     "eph_msystem_load": (I, [C.c_char_p, I, PP]),
     "eph_msystem_destroy": (I, [P]),
     "eph_msystem_size": (I64, [P]),

@@ -123,13 +123,12 @@ __device__ float fzcorr(float x, float y, int kh) {
     const int N = 32;
     x *= (float)N;
     y *= (float)N;
-    int i = (int)floorf(x), j = (int)floorf(y);
-    j -= kh * N;
-    if (i < 0) i = 0;
-    if (j < 0) j = 0;
-    if (i >= N) i = N - 1;
-    if (j >= N) j = N - 1;
-    j += kh * N;
+    // cell indices clamped to the table while still floats, so a non-finite
+    // argument (a system gone bad) still reads inside the table; for finite
+    // arguments this is the integer clamp of ring.impl:2072-2079
+    const float fi = fminf(fmaxf(floorf(x), 0.0f), (float)(N - 1));
+    const float fj = fminf(fmaxf(floorf(y) - (float)(kh * N), 0.0f), (float)(N - 1));
+    const int i = (int)fi, j = (int)fj + kh * N;
     x -= (float)i;
     y -= (float)j;
     const float *p = g_fz + 16 * (2 * N * i + j);
@@ -1260,14 +1259,17 @@ __global__ __launch_bounds__(kPartBS) void k_particles(PView pv, int n_steps, do
                                                        const PBlock *__restrict__ blocks) {
     const int p = blockIdx.x * kPartBS + (int)threadIdx.x;
     int64_t entry0 = 0;
+    int n_valid = kPartBS;
     if (blocks) {                                           // per-workgroup job offset and length
         entry0 = blocks[blockIdx.x].rec_entry0;
         n_steps = blocks[blockIdx.x].n_steps;
+        n_valid = blocks[blockIdx.x].n_valid;
     }
     if (n_steps == 0) return;
     const bool init = n_steps < 0;
-    const bool valid = p < pv.np;
-    const int q = valid ? p : 0;
+    // lanes past the last particle replay the workgroup's first one and store nothing
+    const bool valid = p < pv.np && (int)threadIdx.x < n_valid;
+    const int q = valid ? p : blockIdx.x * kPartBS;
     double *P = pv.P;
     const int64_t S = pv.npp;
     auto ld = [&](int f) { return P[f * S + q]; };

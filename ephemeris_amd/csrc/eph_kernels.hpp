@@ -64,8 +64,9 @@ void launch_np(const View &d, int nrows, int max_n, int maxT, const int2 *recv_b
 void launch_stage(const View &d, int k, double dt, double dt_k, double tadd, hipStream_t st);
 void launch_extrema(const View &d, int invert, hipStream_t st);
 // per-workgroup schedule of a particle launch: first record entry (25 per
-// step, counted from the field's epoch) and number of steps
-struct PBlock { int64_t rec_entry0; int32_t n_steps, pad; };
+// step), number of steps (< 0: first acceleration from entry rec_entry0,
+// 0: idle) and number of particles of the workgroup (>= 1)
+struct PBlock { int64_t rec_entry0; int32_t n_steps, n_valid; };
 // n_steps < 0 computes the first acceleration from record entry 0
 void launch_particles(const PView &pv, int n_steps, double dt, const DtK &dtk, hipStream_t st,
                       const PBlock *blocks = nullptr);

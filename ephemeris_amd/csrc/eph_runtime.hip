@@ -22,12 +22,6 @@ int fail(int code, const std::string &msg) {
     return code;
 }
 
-#define HIP_CHECK(expr)                                                          \
-    do {                                                                         \
-        hipError_t e_ = (expr);                                                  \
-        if (e_ != hipSuccess)                                                    \
-            return fail(EPH_EDEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
-    } while (0)
 
 static const double kRK[325][2] = {
 #include "../tables/rk12_feagin.inc"
@@ -592,6 +586,43 @@ Particles::~Particles() {
 
 static constexpr int64_t kRecChunk = 128;      // steps of field records per chunk
 
+// The partner data (JMain) each body of the one-system batch `field`
+// presents at its current state -- what the last force evaluation left --
+// written to dst[0..kSmallMax) in device memory.
+int field_epoch_record(Batch *field, JMain *dst) {
+    std::vector<JMain> t0(kSmallMax);
+    std::vector<eph_mass> fm(field->nb);
+    int rc = field->download(fm.data());
+    if (rc) return rc;
+    for (int64_t j = 0; j < field->nb; ++j) {
+        const eph_mass &m = fm[j];
+        JMain &T = t0[j];
+        memset(&T, 0, sizeof T);
+        T.rxh = m.r.x.hi; T.rxl = m.r.x.lo; T.ryh = m.r.y.hi; T.ryl = m.r.y.lo; T.rzh = m.r.z.hi; T.rzl = m.r.z.lo;
+        T.GM = m.GM; T.sR2 = m.sR2;
+        T.beta2 = m.beta2;
+        T.bx = m.beta.x; T.by = m.beta.y; T.bz = m.beta.z;
+        T.R2 = m.R2;
+        T.GLx = m.GL.x.hi + m.GL.x.lo; T.GLy = m.GL.y.hi + m.GL.y.lo; T.GLz = m.GL.z.hi + m.GL.z.lo;
+        T.lum = m.lum;
+        const double *C = &m.C_potential.x.x;
+        for (int q = 0; q < 9; ++q) T.C[q] = C[q];
+    }
+    // phi/c^2 and naccel/c^2 exactly as the device holds them
+    const size_t nbp = (size_t)field->nbp;
+    std::vector<double> pc(field->nb), nc(3 * field->nb);
+    HIP_CHECK(hipMemcpy(pc.data(), field->F + F_PHIC * nbp, sizeof(double) * field->nb, hipMemcpyDeviceToHost));
+    for (int c = 0; c < 3; ++c)
+        HIP_CHECK(hipMemcpy(nc.data() + c * field->nb, field->F + (F_NACCC + c) * nbp, sizeof(double) * field->nb,
+                            hipMemcpyDeviceToHost));
+    for (int64_t j = 0; j < field->nb; ++j) {
+        t0[j].phi = pc[j];
+        t0[j].nx = nc[j]; t0[j].ny = nc[field->nb + j]; t0[j].nz = nc[2 * field->nb + j];
+    }
+    HIP_CHECK(hipMemcpy(dst, t0.data(), sizeof(JMain) * kSmallMax, hipMemcpyHostToDevice));
+    return EPH_OK;
+}
+
 int create_particles(Batch *field, int64_t n, const double *r, const double *v, double GM, double R, Particles **out) {
     if (!field || !out || n < 0 || (n && (!r || !v))) return fail(EPH_EINVAL, "eph_particles_create: bad argument");
     if (field->n_sys != 1 || !field->all_small())
@@ -616,41 +647,10 @@ int create_particles(Batch *field, int64_t n, const double *r, const double *v, 
     ps->rec_steps = kRecChunk;
     const size_t rec_bytes = sizeof(JMain) * kSmallMax * (size_t)(25 * kRecChunk);
     if (hipMalloc(&ps->rec, rec_bytes) != hipSuccess) return bail(fail(EPH_ENOMEM, "field record allocation failed"));
-    // initial acceleration of each particle from the field's current state
-    std::vector<JMain> t0(kSmallMax);
     {
-        std::vector<eph_mass> fm(field->nb);
-        int rc = field->download(fm.data());
+        int rc = field_epoch_record(field, ps->rec);
         if (rc) return bail(rc);
-        for (int64_t j = 0; j < field->nb; ++j) {
-            const eph_mass &m = fm[j];
-            JMain &T = t0[j];
-            memset(&T, 0, sizeof T);
-            T.rxh = m.r.x.hi; T.rxl = m.r.x.lo; T.ryh = m.r.y.hi; T.ryl = m.r.y.lo; T.rzh = m.r.z.hi; T.rzl = m.r.z.lo;
-            T.GM = m.GM; T.sR2 = m.sR2;
-            T.phi = m.phi / kC2;              // placeholder, replaced below by the device values
-            T.beta2 = m.beta2;
-            T.bx = m.beta.x; T.by = m.beta.y; T.bz = m.beta.z;
-            T.R2 = m.R2;
-            T.GLx = m.GL.x.hi + m.GL.x.lo; T.GLy = m.GL.y.hi + m.GL.y.lo; T.GLz = m.GL.z.hi + m.GL.z.lo;
-            T.lum = m.lum;
-            const double *C = &m.C_potential.x.x;
-            for (int q = 0; q < 9; ++q) T.C[q] = C[q];
-        }
-        // phi/c^2 and naccel/c^2 exactly as the last force evaluation left them
-        std::vector<double> pc(field->nb), nc(3 * field->nb);
-        for (int64_t j = 0; j < field->nb; ++j) {
-            if (hipMemcpy(&pc[j], field->F + (size_t)F_PHIC * field->nbp + j, 8, hipMemcpyDeviceToHost) != hipSuccess)
-                return bail(fail(EPH_EDEVICE, "field read failed"));
-            for (int c = 0; c < 3; ++c)
-                if (hipMemcpy(&nc[3 * j + c], field->F + (size_t)(F_NACCC + c) * field->nbp + j, 8, hipMemcpyDeviceToHost) != hipSuccess)
-                    return bail(fail(EPH_EDEVICE, "field read failed"));
-            t0[j].phi = pc[j];
-            t0[j].nx = nc[3 * j]; t0[j].ny = nc[3 * j + 1]; t0[j].nz = nc[3 * j + 2];
-        }
     }
-    if (hipMemcpy(ps->rec, t0.data(), sizeof(JMain) * kSmallMax, hipMemcpyHostToDevice) != hipSuccess)
-        return bail(fail(EPH_EDEVICE, "record upload failed"));
     PView pv{ps->P, ps->npp, (int)ps->np, ps->rec, 1, (int)field->nb, R < 0 ? R * R : 0.0};
     launch_particles(pv, -1, 0.0, DtK{}, field->stream);
     if (hipStreamSynchronize(field->stream) != hipSuccess) return bail(fail(EPH_EDEVICE, "particle init failed"));

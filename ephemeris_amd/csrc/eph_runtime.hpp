@@ -11,6 +11,14 @@ namespace eph {
 
 int fail(int code, const std::string &msg);
 
+// return the library status of a failed HIP call from the enclosing function
+#define HIP_CHECK(expr)                                                          \
+    do {                                                                         \
+        hipError_t e_ = (expr);                                                  \
+        if (e_ != hipSuccess)                                                    \
+            return fail(EPH_EDEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
 constexpr int kNumKernelClasses = 7;
 constexpr int P_COUNT_HOST = 17;     // particle fields (PField)
 extern const char *const kKernelNames[kNumKernelClasses];
@@ -112,6 +120,7 @@ DtK stage_dtk();
 int regroup_batch(Batch *old, const std::vector<int> &keep, int64_t n_new, const int64_t *new_sys_n,
                   const eph_mass *new_masses, const eph_geopotential *new_gps, int64_t n_new_gp,
                   const eph_ring *new_rings, int64_t n_new_ring, const double *new_t_eph, Batch **out);
+int field_epoch_record(Batch *field, JMain *dst);
 int create_particles(Batch *field, int64_t n, const double *r, const double *v, double GM, double R, Particles **out);
 int integrate_particles(Particles *ps, double dt, int64_t n_step);
 int download_particles(Particles *ps, double *r, double *v, double *extrema);

@@ -153,6 +153,74 @@ int eph_particles_download(eph_particles *ps, double *r, double *v, double *extr
 int64_t eph_particles_count(const eph_particles *ps);
 int eph_particles_destroy(eph_particles *ps);
 
+/* ---- serving -----------------------------------------------------------------
+ * A request loop over resident state: jobs are submitted at any time (from
+ * any thread), admitted first-in first-out at tick boundaries while their
+ * estimated HBM footprint fits the budget and the device's free memory, and
+ * advanced together -- every admitted ensemble system in one batch, every
+ * admitted particle set in one launch over a shared pool -- until their span
+ * is done; state moves between batches device to device.
+ *  - Ensemble job: n_sys independent systems (model indices local to the
+ *    job), n_steps RK12 steps of cfg.ens_dt each, taken in calls of
+ *    cfg.ens_quantum steps (n_steps must be a multiple of it): the result is
+ *    bitwise that of eph_integrate(sys, ens_dt, ens_quantum) repeated.
+ *  - Particle job: n particles of radius cfg.part_R in the server's field
+ *    system from its epoch, n_steps steps of cfg.part_dt (at most
+ *    cfg.part_max_steps): bitwise that of eph_particles_create on a copy of
+ *    the field followed by eph_particles_integrate(ps, part_dt, n_steps).
+ * Jobs that fail validation at submission are rejected (EPH_EINVAL /
+ * EPH_ESANITY, no job id); a job whose state turns non-finite ends FAILED
+ * without touching other jobs, whose results never depend on what else is
+ * resident.  Times are seconds on a steady clock since eph_server_create.
+ * eph_server_tick runs one tick on the calling thread; eph_server_start runs
+ * ticks on a worker thread until eph_server_stop (eph_server_tick then
+ * returns EPH_EBUSY).
+ * No counterpart in the reference, which integrates one system per process
+ * (src/modules/ephemeris_generator.cpp:130-166). */
+typedef struct eph_server eph_server;
+typedef struct {
+    int64_t mem_budget;      /* bytes for resident jobs; 0 = 40% of free device memory */
+    double ens_dt;           /* ensemble step (s) */
+    int64_t ens_quantum;     /* ensemble steps per tick */
+    double part_dt;          /* particle step (s) */
+    int64_t part_quantum;    /* particle steps per tick */
+    int64_t part_max_steps;  /* longest particle span (field records are kept for it) */
+    double part_GM, part_R;  /* particle gravitational parameter and radius */
+    int64_t max_systems;     /* ensemble slots; 0 = unlimited */
+    int64_t max_particles;   /* particle slots; 0 = unlimited */
+} eph_server_config;
+enum { EPH_JOB_QUEUED = 0, EPH_JOB_RUNNING = 1, EPH_JOB_DONE = 2, EPH_JOB_FAILED = 3 };
+enum { EPH_JOB_ENSEMBLE = 0, EPH_JOB_PARTICLES = 1 };
+typedef struct {
+    int32_t kind, status;
+    int64_t steps_total, steps_done, bytes, units;   /* units: bodies or particles */
+    double t_submit, t_admit, t_done;                 /* -1 until reached */
+    char error[160];
+} eph_job_info;
+typedef struct {
+    int64_t ticks, jobs_submitted, jobs_done, jobs_failed, jobs_queued, jobs_running;
+    int64_t mem_used, mem_peak, mem_budget;
+    double pair_interactions, rk12_system_steps, particle_steps, busy_s, t_now;
+} eph_server_stats;
+/* field: one system of at most 48 bodies (copied; NULL = no particle jobs) */
+int eph_server_create(const eph_server_config *cfg, eph_system *field, eph_server **out);
+int eph_server_submit_ensemble(eph_server *srv, int64_t n_sys, const int64_t *sys_n, const eph_mass *masses,
+                               const eph_geopotential *gps, int64_t n_gp, const eph_ring *rings,
+                               int64_t n_ring, const double *t_eph, int64_t n_steps, int64_t *job);
+int eph_server_submit_particles(eph_server *srv, int64_t n, const double *r, const double *v,
+                                int64_t n_steps, int64_t *job);
+int eph_server_tick(eph_server *srv, int64_t *n_active);
+int eph_server_start(eph_server *srv);
+int eph_server_wait(eph_server *srv, double timeout_s);     /* until no job is queued or running */
+int eph_server_stop(eph_server *srv);
+int eph_server_job_info(eph_server *srv, int64_t job, eph_job_info *info);
+/* results of a DONE job, layouts as eph_system_download / eph_particles_download */
+int eph_server_result_ensemble(eph_server *srv, int64_t job, eph_mass *masses, double *t_eph);
+int eph_server_result_particles(eph_server *srv, int64_t job, double *r, double *v, double *extrema);
+int eph_server_release(eph_server *srv, int64_t job);       /* forget a finished job */
+int eph_server_stats_read(eph_server *srv, eph_server_stats *st);
+int eph_server_destroy(eph_server *srv);
+
 /* ---- measurement ---------------------------------------------------------- */
 /* Average device time (ms) per launch of each kernel class, measured with
  * HIP events on the library's stream since the last reset; names[] gets
