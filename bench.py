@@ -104,6 +104,71 @@ def particles_leg(n_particles, steps):
             "kernel_ms": {k: round(v[0], 4) for k, v in prof.items() if v[1]}}
 
 
+def serving_leg(n_jobs, rate, seed, verify):
+    """Poisson arrivals of particle and ensemble jobs into the server."""
+    import tempfile
+    from ephemeris_amd import generator as gen
+    from ephemeris_amd.particles import ParticleSet
+    from ephemeris_amd.server import Server
+    from ephemeris_amd.serving import ClonePool, latency_summary, make_mix, serve
+    from ephemeris_amd.system import GpuSystems, MSystem
+    from tests.util import bits_equal
+    f9 = gen.make_field9(gen.make_full())
+    cfg = gen.write_system(f9, tempfile.mkdtemp(prefix="eph_f9_"), "Field9")
+
+    def field():
+        m = MSystem(cfg)
+        gps, rings = m.models()
+        f = GpuSystems(m.masses, None, gps, rings, [m.t_eph()])
+        m.close()
+        return f
+
+    t0 = time.perf_counter()
+    pool = ClonePool(32)
+    jobs = make_mix(n_jobs, rate, seed=seed, pool_size=32)
+    npart = max([j.n for j in jobs if j.kind == "particles"] + [1])
+    r, v = gen.make_particles(max(npart, 1000), seed=seed, field9=f9)
+    srv = Server(field(), ens_quantum=288, part_quantum=16, part_max_steps=365)
+    t_setup = time.perf_counter() - t0
+    recs, makespan = serve(srv, jobs, pool, (r, v), progress=lambda m: print(m, file=sys.stderr, flush=True))
+    st = srv.stats()
+    out = {"jobs": n_jobs, "arrival_rate_per_s": rate, "seed": seed, "setup_s": t_setup, "makespan_s": makespan,
+           "latency": latency_summary(recs),
+           "pair_interactions_per_s": st["pair_interactions"] / makespan,
+           "rk12_system_steps_per_s": st["rk12_system_steps"] / makespan,
+           "particle_steps_per_s": st["particle_steps"] / makespan,
+           "jobs_per_s": sum(x["status"] == "done" for x in recs) / makespan,
+           "ticks": st["ticks"], "busy_s": st["busy_s"], "mem_peak_bytes": st["mem_peak"],
+           "mem_budget_bytes": st["mem_budget"]}
+    if verify:
+        # the smallest job of each kind again, alone through the plain entry points
+        checked = ok = 0
+        pj = [(x, j) for x, j in zip(recs, jobs) if j.kind == "particles" and x["status"] == "done"]
+        if pj:
+            x, j = min(pj, key=lambda t: t[0]["units"])
+            off = x["offset"]
+            ps = ParticleSet(field(), r[off:off + j.n], v[off:off + j.n])
+            ps.integrate(86400.0, j.days)
+            a = ps.download()
+            b = srv.result_particles(x["job"])
+            checked += 1
+            ok += all(bits_equal(p, q) for p, q in zip(a, b))
+        ej = [(x, j) for x, j in zip(recs, jobs) if j.kind == "ensemble" and x["status"] == "done"]
+        if ej:
+            x, j = min(ej, key=lambda t: t[0]["units"] * t[0]["days"])
+            m, sn, g, rg, t = pool.job_arrays(j.members)
+            s = GpuSystems(m, sn, g, rg, t)
+            for _ in range(j.days):
+                s.integrate(300.0, 288)
+            a, ta = s.download()
+            b, tb = srv.result_ensemble(x["job"])
+            checked += 1
+            ok += bits_equal(a, b) and bits_equal(ta, tb)
+        out["verified_alone_vs_mix"] = f"{ok}/{checked} bit-identical"
+    srv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +182,10 @@ def main():
     ap.add_argument("--particles", type=int, default=1000000)
     ap.add_argument("--particle-steps", type=int, default=4)
     ap.add_argument("--no-particles", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=30)
+    ap.add_argument("--serve-rate", type=float, default=0.5, help="job arrivals per second (Poisson)")
+    ap.add_argument("--serve-seed", type=int, default=20254)
+    ap.add_argument("--no-serve", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported (each system, clone or particle "
@@ -162,6 +231,8 @@ def main():
         out["batched"] = b
     if not a.no_particles:
         out["particles"] = particles_leg(a.particles, a.particle_steps)
+    if not a.no_serve:
+        out["serving"] = serving_leg(a.serve_jobs, a.serve_rate, a.serve_seed, verify=True)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, a.cpu_steps)
     print(json.dumps(out))

@@ -85,7 +85,7 @@ def serve(server, jobs: List[JobSpec], pool: ClonePool, particles, time_scale: f
     r_all, v_all = particles
     server.start()
     t0 = time.perf_counter()
-    ids = []
+    ids, offs = [], []
     off = 0
     for js in jobs:
         wait = js.arrival * time_scale - (time.perf_counter() - t0)
@@ -95,10 +95,12 @@ def serve(server, jobs: List[JobSpec], pool: ClonePool, particles, time_scale: f
             if off + js.n > len(r_all):
                 off = 0
             jid = server.submit_particles(r_all[off:off + js.n], v_all[off:off + js.n], js.days)
+            offs.append(off)
             off += js.n
         else:
             m, sn, g, r, t = pool.job_arrays(js.members)
             jid = server.submit_ensemble(m, sn, g, r, t, n_steps=js.days * 288)
+            offs.append(-1)
         ids.append(jid)
         if progress:
             progress(f"submitted job {jid} ({js.kind}) at {time.perf_counter() - t0:.1f}s")
@@ -106,9 +108,9 @@ def serve(server, jobs: List[JobSpec], pool: ClonePool, particles, time_scale: f
     makespan = time.perf_counter() - t0
     server.stop()
     recs = []
-    for js, jid in zip(jobs, ids):
+    for js, jid, o in zip(jobs, ids, offs):
         i = server.info(jid)
-        recs.append(dict(job=jid, kind=js.kind, days=js.days, units=i["units"], status=i["status_name"],
+        recs.append(dict(job=jid, kind=js.kind, days=js.days, units=i["units"], status=i["status_name"], offset=o,
                          latency=i["t_done"] - i["t_submit"], queued=i["t_admit"] - i["t_submit"],
                          error=i["error"]))
     return recs, makespan
