@@ -593,24 +593,55 @@ __global__ __launch_bounds__(BS) void k_main_pool(View d, const int2 *__restrict
 // a 64-partner tile and leaves that pair's terms in LDS; then one lane per
 // accumulator adds them in partner order, so every sum is formed exactly as
 // in the in-order loop above while a 504-body system occupies 504 waves.
+//
+// The accumulating lanes run one common, branch-free instruction stream:
+// every lane adds kWpbSteps terms per partner (its own chain's terms, padded
+// with +0), and the self pair and partners past the end contribute +0.  A
+// sum that starts at +0 never holds -0 in round-to-nearest, so adding +0 is
+// an exact identity; a - b is a + (-b) exactly, so subtracted terms are
+// stored negated.  The running maxima do not depend on order (ties are equal
+// values, NaN wins either way) and are reduced across the wave at the end.
 // ---------------------------------------------------------------------------
-constexpr int kWpbTerms = 26;
+constexpr int kWpbRows = 25;          // 24 term rows + one row of zeros
+constexpr int kWpbPitch = 65;         // row pitch in doubles: rows start in different LDS banks
+constexpr int kWpbSteps = 5;
+constexpr int kWpbZero = kWpbRows - 1;
 
-// main force (thread-pool order, mass.cpp:147-155): accumulators
-// 0-2 gaccel, 3-5 daccel, 6-8 dtorque, 9 min_distance, 10 max_influence
+// row of the k-th term added by accumulator lane `lane` (main force,
+// thread-pool order, mass.cpp:147-155): lanes 0-2 gaccel (+G), 3-5 daccel
+// (+own quadrupole, +partner's quadrupole, -LT own spin, +LT partner's spin,
+// +radiation), 6-8 dtorque (+r x dg, -LT torque)
+__device__ __forceinline__ int wpb_main_row(int lane, int k) {
+    const int c = lane % 3;
+    if (lane < 3) return k == 0 ? c : kWpbZero;
+    if (lane < 6) return 3 + 3 * k + c;
+    if (lane < 9) return k < 2 ? 18 + 3 * k + c : kWpbZero;
+    return kWpbZero;
+}
+
+__device__ __forceinline__ double wave_checked_max(double x) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) x = checked_max_into(x, __shfl_xor(x, m, 64));
+    return x;
+}
+
 __global__ __launch_bounds__(64) void k_main_wpb(View d, const int2 *__restrict__ work) {
     const Acc A{d.F, d.nbp};
     const int s = work[blockIdx.x].x, il = work[blockIdx.x].y;
     const SysMeta sm = d.sys[s];
     const int n = sm.n, off = sm.off, gi = off + il;
     const int lane = (int)threadIdx.x;
-    __shared__ double term[kWpbTerms][64];
+    __shared__ double term[kWpbRows * kWpbPitch];
     const IMain I = load_imain(A, gi);
+    term[kWpbZero * kWpbPitch + lane] = 0.0;
+    int row[kWpbSteps];
+#pragma unroll
+    for (int k = 0; k < kWpbSteps; ++k) row[k] = wpb_main_row(lane, k) * kWpbPitch;
     double acc = 0;
-    if (lane == 9) acc = A.ld(F_MIND, gi);
-    if (lane == 10) acc = A.ld(F_MAXI, gi);
+    double mind = 0, maxi = 0;                             // lane-local maxima (>= +0: identities)
     for (int t0 = 0; t0 < n; t0 += 64) {
         const int jl = t0 + lane;
+        double t[kWpbZero];
         if (jl < n && jl != il) {
             JMain T;
             load_jmain(A, T, off + jl);
@@ -627,45 +658,35 @@ __global__ __launch_bounds__(64) void k_main_wpb(View d, const int2 *__restrict_
             const v3 GLj{T.GLx, T.GLy, T.GLz};
             const v3 lt3 = cross(rcr32 * (GLj - (3 * dot(GLj, p.r) * p.rr2) * p.r), p.b);
             const v3 rad = (T.lum * I.rR2G4c * p.rr3 / I.GM) * p.r;
-            const double t[kWpbTerms] = {p.G.x, p.G.y, p.G.z, dg.x, dg.y, dg.z, gj.x, gj.y, gj.z,
-                                         lt1.x, lt1.y, lt1.z, lt3.x, lt3.y, lt3.z, rad.x, rad.y, rad.z,
-                                         rxdg.x, rxdg.y, rxdg.z, lt2.x, lt2.y, lt2.z, p.rr, p.tp_dg};
+            const double v[kWpbZero] = {p.G.x, p.G.y, p.G.z, dg.x, dg.y, dg.z, gj.x, gj.y, gj.z,
+                                        -lt1.x, -lt1.y, -lt1.z, lt3.x, lt3.y, lt3.z, rad.x, rad.y, rad.z,
+                                        rxdg.x, rxdg.y, rxdg.z, -lt2.x, -lt2.y, -lt2.z};
 #pragma unroll
-            for (int q = 0; q < kWpbTerms; ++q) term[q][lane] = t[q];
+            for (int q = 0; q < kWpbZero; ++q) t[q] = v[q];
+            mind = checked_max_into(mind, p.rr);
+            maxi = checked_max_into(maxi, p.tp_dg);
+        } else {
+#pragma unroll
+            for (int q = 0; q < kWpbZero; ++q) t[q] = 0.0;
         }
+#pragma unroll
+        for (int q = 0; q < kWpbZero; ++q) term[q * kWpbPitch + lane] = t[q];
         __syncthreads();
-        const int tn = min(64, n - t0);
-        if (lane < 3) {
-            for (int q = 0; q < tn; ++q)
-                if (t0 + q != il) acc += term[lane][q];
-        } else if (lane < 6) {
-            const int c = lane - 3;
-            for (int q = 0; q < tn; ++q) {
-                if (t0 + q == il) continue;
-                acc += term[3 + c][q];      // own quadrupole
-                acc += term[6 + c][q];      // partner's quadrupole
-                acc -= term[9 + c][q];      // Lense-Thirring, own spin
-                acc += term[12 + c][q];     // Lense-Thirring, partner's spin
-                acc += term[15 + c][q];     // radiation
-            }
-        } else if (lane < 9) {
-            const int c = lane - 6;
-            for (int q = 0; q < tn; ++q) {
-                if (t0 + q == il) continue;
-                acc += term[18 + c][q];
-                acc -= term[21 + c][q];
-            }
-        } else if (lane < 11) {
-            for (int q = 0; q < tn; ++q)
-                if (t0 + q != il) acc = checked_max_into(acc, term[24 + (lane - 9)][q]);
+        if (lane < 9) {
+#pragma unroll 8
+            for (int q = 0; q < 64; ++q)
+#pragma unroll
+                for (int k = 0; k < kWpbSteps; ++k) acc += term[row[k] + q];
         }
         __syncthreads();
     }
+    mind = checked_max_into(A.ld(F_MIND, gi), wave_checked_max(mind));
+    maxi = checked_max_into(A.ld(F_MAXI, gi), wave_checked_max(maxi));
     if (lane < 3) A.st(F_GACC + lane, gi, acc);
     else if (lane < 6) A.st(F_DACC + lane - 3, gi, acc);
     else if (lane < 9) A.st(F_DTQ + lane - 6, gi, acc);
-    else if (lane == 9) A.st(F_MIND, gi, acc);
-    else if (lane == 10) A.st(F_MAXI, gi, acc);
+    else if (lane == 9) A.st(F_MIND, gi, mind);
+    else if (lane == 10) A.st(F_MAXI, gi, maxi);
     if (sm.nitem == 0 && lane == 11) {                     // FINALIZE_RELATIVITY
         A.st(F_PHI, gi, I.phi * kC2);
         A.stv(F_NACC, gi, A.ldv(F_NACCC, gi) * kC2);
@@ -673,14 +694,17 @@ __global__ __launch_bounds__(64) void k_main_wpb(View d, const int2 *__restrict_
 }
 
 // deform + spin fixed point + PREPARE_RELATIVITY, one wavefront per body.
-// Accumulators: lane 0 phi, 1-3 naccel, 4-12 the tidal matrix.
+// Accumulators: lane 0 phi, 1-3 naccel, 4-12 the tidal matrix; the
+// subtracted terms (phi, tidal matrix) are stored negated and the self pair
+// contributes +0, as in k_main_wpb.
+constexpr int kWpbDefPitch = 65;
 __global__ __launch_bounds__(64) void k_deform_wpb(View d, const int2 *__restrict__ work, int prepare) {
     const Acc A{d.F, d.nbp};
     const int s = work[blockIdx.x].x, il = work[blockIdx.x].y;
     const SysMeta sm = d.sys[s];
     const int n = sm.n, off = sm.off, gi = off + il;
     const int lane = (int)threadIdx.x;
-    __shared__ double term[13][64];
+    __shared__ double term[13 * kWpbDefPitch];
     __shared__ double sums[13];
     __shared__ int again_s;
     const ddv3 ri = A.lddv(F_R, gi), vi = A.lddv(F_V, gi);
@@ -695,6 +719,9 @@ __global__ __launch_bounds__(64) void k_deform_wpb(View d, const int2 *__restric
         double acc = 0;
         for (int t0 = 0; t0 < n; t0 += 64) {
             const int jl = t0 + lane;
+            double t[13];
+#pragma unroll
+            for (int q = 0; q < 13; ++q) t[q] = 0.0;
             if (jl < n && jl != il) {
                 const int gj = off + jl;
                 const v3 r = diff(A.lddv(F_R, gj), ri);
@@ -703,14 +730,14 @@ __global__ __launch_bounds__(64) void k_deform_wpb(View d, const int2 *__restric
                 const double tp_dphi = rr * A.ld(F_GM, gj);
                 const double tp_dg = rr2 * tp_dphi;
                 const v3 na = tp_dg * r;
-                term[0][lane] = tp_dphi;
-                term[1][lane] = na.x; term[2][lane] = na.y; term[3][lane] = na.z;
+                t[0] = -tp_dphi;
+                t[1] = na.x; t[2] = na.y; t[3] = na.z;
                 if (ndelay) {
                     const double fmj = -3 * tp_dg;
-                    const m3 t = (fmj * k2) * subdiag(outer(rr2 * r, r), 1.0 / 3);
-                    const double *tp = &t.x.x;
+                    const m3 tm = (fmj * k2) * subdiag(outer(rr2 * r, r), 1.0 / 3);
+                    const double *tp = &tm.x.x;
 #pragma unroll
-                    for (int q = 0; q < 9; ++q) term[4 + q][lane] = tp[q];
+                    for (int q = 0; q < 9; ++q) t[4 + q] = -tp[q];
                 } else if (damped) {
                     const v3 v = diff(A.lddv(F_V, gj), vi);
                     const double fmj = -3 * tp_dg;
@@ -723,24 +750,20 @@ __global__ __launch_bounds__(64) void k_deform_wpb(View d, const int2 *__restric
                                  y0 = dwt * sqrt(2 / (ecwt2 * (1 + secwt2))),
                                  z02 = dwt * dwt * (2 / (ecwt2 + secwt2));
                     const v3 dr = x0 * r + y0 * cross(dw, r);
-                    m3 t = outer(rr2 * dr, dr) - outer(z02 * dw, dw);
-                    t = (fmj * k2) * adddiag(t, z02 - 1.0 / 3);
-                    const double *tp = &t.x.x;
+                    m3 tm = outer(rr2 * dr, dr) - outer(z02 * dw, dw);
+                    tm = (fmj * k2) * adddiag(tm, z02 - 1.0 / 3);
+                    const double *tp = &tm.x.x;
 #pragma unroll
-                    for (int q = 0; q < 9; ++q) term[4 + q][lane] = tp[q];
+                    for (int q = 0; q < 9; ++q) t[4 + q] = -tp[q];
                 }
             }
+#pragma unroll
+            for (int q = 0; q < 13; ++q) term[q * kWpbDefPitch + lane] = t[q];
             __syncthreads();
-            const int tn = min(64, n - t0);
-            if (lane == 0) {
-                for (int q = 0; q < tn; ++q)
-                    if (t0 + q != il) acc -= term[0][q];
-            } else if (lane < 4) {
-                for (int q = 0; q < tn; ++q)
-                    if (t0 + q != il) acc += term[lane][q];
-            } else if (lane < nacc_chains) {
-                for (int q = 0; q < tn; ++q)
-                    if (t0 + q != il) acc -= term[lane][q];
+            if (lane < nacc_chains) {
+                const double *row = term + lane * kWpbDefPitch;
+#pragma unroll 16
+                for (int q = 0; q < 64; ++q) acc += row[q];
             }
             __syncthreads();
         }

@@ -249,8 +249,16 @@ int Batch::accel(int mode, bool prepare_only) {
     const bool wpb = use_wpb(mode);
     {
         Scope sc(profiler, 1, stream);
-        if (wpb) launch_deform_wpb(v, d_work_all, n_work_all, prepare_only ? 0 : 1, stream);
-        else launch_deform(v, d_blk_light, n_blk_light, d_blk_damped, n_blk_damped, prepare_only ? 0 : 1, stream);
+        if (wpb) {
+            launch_deform_wpb(v, d_work_all, n_work_all, prepare_only ? 0 : 1, stream);
+        } else if (variant == 1) {
+            launch_deform(v, d_blk_light, n_blk_light, d_blk_damped, n_blk_damped, prepare_only ? 0 : 1, stream);
+        } else {
+            // constant-time-lag bodies re-sum their partners once per spin pass:
+            // one wavefront each, the rest one thread each
+            launch_deform_wpb(v, d_work_damped, n_work_damped, prepare_only ? 0 : 1, stream);
+            launch_deform(v, d_blk_light, n_blk_light, nullptr, 0, prepare_only ? 0 : 1, stream);
+        }
     }
     if (prepare_only) return EPH_OK;
     if (pl.n_pool) {
@@ -357,6 +365,7 @@ int Batch::integrate(double dt, int64_t n_step, int mode) {
 int Batch::build_class_tables(const eph_mass *masses) {
     std::vector<int32_t> perm(nb);
     std::vector<int4> light, damped;
+    std::vector<int2> damped_work;       // (system, body) of every constant-time-lag body
     for (int64_t s = 0; s < n_sys; ++s) {
         int64_t k = sys_off[s];
         for (int pass = 0; pass < 2; ++pass)
@@ -365,7 +374,11 @@ int Batch::build_class_tables(const eph_mass *masses) {
                 if (dmp == (pass == 0)) perm[k++] = (int32_t)g;
             }
         int64_t nd = 0;
-        for (int64_t g = sys_off[s]; g < sys_off[s + 1]; ++g) nd += masses[g].k2 != 0 && masses[g].tide_delay != 0;
+        for (int64_t g = sys_off[s]; g < sys_off[s + 1]; ++g)
+            if (masses[g].k2 != 0 && masses[g].tide_delay != 0) {
+                ++nd;
+                damped_work.push_back(make_int2((int)s, (int)(g - sys_off[s])));
+            }
         for (int64_t q = 0; q < nd; q += kBS) damped.push_back(make_int4((int)s, (int)q, (int)std::min<int64_t>(kBS, nd - q), 0));
         for (int64_t q = nd; q < sys_n[s]; q += kBS)
             light.push_back(make_int4((int)s, (int)q, (int)std::min<int64_t>(kBS, sys_n[s] - q), 0));
@@ -385,8 +398,10 @@ int Batch::build_class_tables(const eph_mass *masses) {
     if ((rc = up(d_perm, perm))) return rc;
     if ((rc = up(d_blk_light, light))) return rc;
     if ((rc = up(d_blk_damped, damped))) return rc;
+    if ((rc = up(d_work_damped, damped_work))) return rc;
     n_blk_light = (int)light.size();
     n_blk_damped = (int)damped.size();
+    n_work_damped = (int)damped_work.size();
     return EPH_OK;
 }
 

@@ -75,6 +75,8 @@ struct Batch {
     int32_t *d_perm = nullptr;
     int4 *d_blk_light = nullptr, *d_blk_damped = nullptr;
     int n_blk_light = 0, n_blk_damped = 0;
+    int2 *d_work_damped = nullptr;
+    int n_work_damped = 0;
     std::vector<void *> class_allocs;
     Plan plan[2];
     std::vector<void *> dev_allocs;

@@ -17,6 +17,7 @@ from typing import List, Optional
 
 import numpy as np
 
+from . import _lib
 from . import generator as gen
 from .ensemble import load_members, write_members
 
@@ -104,7 +105,18 @@ def serve(server, jobs: List[JobSpec], pool: ClonePool, particles, time_scale: f
         ids.append(jid)
         if progress:
             progress(f"submitted job {jid} ({js.kind}) at {time.perf_counter() - t0:.1f}s")
-    server.wait(timeout)
+    deadline = time.perf_counter() + timeout
+    while True:                              # wait, reporting progress every 10 s
+        try:
+            server.wait(min(10.0, max(deadline - time.perf_counter(), 0.01)))
+            break
+        except _lib.EphError as e:
+            if e.code != _lib.EPH_EBUSY or time.perf_counter() > deadline:
+                raise
+            if progress:
+                st = server.stats()
+                progress(f"t={time.perf_counter() - t0:.0f}s running={st['jobs_running']} "
+                         f"queued={st['jobs_queued']} done={st['jobs_done']} ticks={st['ticks']}")
     makespan = time.perf_counter() - t0
     server.stop()
     recs = []
