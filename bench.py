@@ -186,7 +186,19 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=0.5, help="job arrivals per second (Poisson)")
     ap.add_argument("--serve-seed", type=int, default=20254)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device 5 times (calibrates HBM byte counters) and exit")
     a = ap.parse_args()
+    if a.copy_probe:
+        import torch
+        x = torch.empty(a.copy_probe * 2 ** 20, dtype=torch.uint8, device="cuda:0")
+        y = torch.empty_like(x)
+        x.fill_(1)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": x.numel(), "copies": 5}))
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported (each system, clone or particle "
               "set fits one MI355X; see DESIGN.md for how the work would shard)", file=sys.stderr)

@@ -240,14 +240,14 @@ __device__ v3 nonpoint_field(const View &d, const Acc &A, int p, int kind, int m
 
 // FINALIZE_RELATIVITY and RING_CORRECTION (mass.impl:133-152) of body g with
 // its final daccel/dtorque; writes phi, naccel, daccel, dtorque
-__device__ void finalize_body(const View &d, const Acc &A, int g, v3 da, v3 dq) {
+__device__ void finalize_with(const View &d, const Acc &A, int g, v3 ga, v3 da, v3 dq) {
     A.st(F_PHI, g, A.ld(F_PHIC, g) * kC2);
     const v3 nacc = A.ldv(F_NACCC, g) * kC2;
     A.stv(F_NACC, g, nacc);
     const int rg = d.np_ring[g];
     if (rg >= 0) {
         const eph_ring &mr = d.rings[rg];
-        da = da - mr.GM_ratio * ((A.ldv(F_GACC, g) + da) + nacc);
+        da = da - mr.GM_ratio * ((ga + da) + nacc);
         v3 mGL = todbl(A.lddv(F_GL, g));
         const double rmGL = normalize(mGL);
         const v3 ptorque = dot(dq, mGL) * mGL;
@@ -257,6 +257,9 @@ __device__ void finalize_body(const View &d, const Acc &A, int g, v3 da, v3 dq) 
     }
     A.stv(F_DACC, g, da);
     A.stv(F_DTQ, g, dq);
+}
+__device__ void finalize_body(const View &d, const Acc &A, int g, v3 da, v3 dq) {
+    finalize_with(d, A, g, A.ldv(F_GACC, g), da, dq);
 }
 
 // ---------------------------------------------------------------------------
@@ -625,6 +628,97 @@ __device__ __forceinline__ double wave_checked_max(double x) {
     return x;
 }
 
+constexpr int kNpFusedMaxN = 1024;    // systems up to this size take the fused non-point pass
+constexpr int kNpFusedMaxItems = 64;
+
+// Non-point pass of body i inside its own wavefront (k_np_field, k_np_item
+// and k_np_recv for one receiver): receiver terms in round order, then the
+// sums of i's own items over partner slices, then finalisation.  `acc` of
+// lanes 3-5 holds daccel, of lanes 6-8 dtorque; `ga` is gaccel.
+__device__ __forceinline__ void wpb_nonpoint(const View &d, const Acc &A, const SysMeta &sm, int il, int gi,
+                                             double &acc, const v3 &ga) {
+    const int lane = (int)threadIdx.x;
+    const int n = sm.n, nit = sm.nitem, T = sm.T;
+    __shared__ double rf[kNpFusedMaxItems * 3];
+    __shared__ double an[3 * kNpFusedMaxN];
+    __shared__ double part[kNpFusedMaxItems * 6];
+    __shared__ double fin[6];
+    __shared__ int own_q[2];
+    const ddv3 ri = A.lddv(F_R, gi);
+    if (lane < 2) own_q[lane] = -1;
+    __syncthreads();
+    // (a) field of every item at r_i (mass.cpp:283-297, receiver side)
+    if (lane < nit) {
+        const NPItem it = d.items[sm.item0 + lane];
+        v3 f{0, 0, 0};
+        if (it.body != gi) f = nonpoint_field(d, A, it.body, it.kind, it.model, diff(ri, A.lddv(F_R, it.body)));
+        else own_q[it.kind] = lane;
+        rf[3 * lane] = f.x; rf[3 * lane + 1] = f.y; rf[3 * lane + 2] = f.z;
+    }
+    __syncthreads();
+    if (lane >= 3 && lane < 6) {
+        const int c = lane - 3;
+        int sj = 0;
+        while ((sj + 1) * n / T <= il) ++sj;
+        for (int k = 0; k < T; ++k) {
+            int cc = sj - k;
+            if (cc < 0) cc += T;
+            const int first = cc * nit / T, last = (cc + 1) * nit / T;
+            for (int q = last - 1; q >= first; --q) {
+                const int pb = d.items[sm.item0 + q].body;
+                if (pb == gi) continue;
+                acc += A.ld(F_GM, pb) * rf[3 * q + c];
+            }
+        }
+    }
+    // (b) own items: geopotential (idaccel) first, then ring (jdaccel), mass.cpp:298-307
+    for (int kind = 0; kind < 2; ++kind) {
+        const int q = own_q[kind];
+        if (q < 0) continue;
+        const NPItem it = d.items[sm.item0 + q];
+        for (int t0 = 0; t0 < n; t0 += 64) {
+            const int jl = t0 + lane;
+            if (jl >= n) break;
+            v3 f{0, 0, 0};
+            if (jl != il) f = nonpoint_field(d, A, gi, kind, it.model, diff(A.lddv(F_R, sm.off + jl), ri));
+            an[3 * jl] = f.x; an[3 * jl + 1] = f.y; an[3 * jl + 2] = f.z;
+        }
+        __syncthreads();
+        if (lane < T) {                                    // round `lane` visits slice (chain + lane) mod T
+            int kk = it.chain + lane;
+            if (kk >= T) kk -= T;
+            const int j0 = kk * n / T, j1 = (kk + 1) * n / T;
+            v3 td{0, 0, 0}, tq{0, 0, 0};
+            for (int jl = j0; jl < j1; ++jl) {
+                if (jl == il) continue;
+                const int gj = sm.off + jl;
+                const v3 r = diff(A.lddv(F_R, gj), ri);
+                const v3 a{an[3 * jl], an[3 * jl + 1], an[3 * jl + 2]};
+                const double GMj = A.ld(F_GM, gj);
+                td -= GMj * a;
+                tq -= GMj * cross(r, a);
+            }
+            double *o = part + 6 * lane;
+            o[0] = td.x; o[1] = td.y; o[2] = td.z; o[3] = tq.x; o[4] = tq.y; o[5] = tq.z;
+        }
+        __syncthreads();
+        if (lane >= 3 && lane < 9) {
+            const int col = lane - 3;                      // 0-2 acceleration, 3-5 torque
+            double sum = 0;
+            for (int k = 0; k < T; ++k) sum += part[6 * k + col];
+            const int f = (kind == 0 ? (col < 3 ? F_IDACC : F_IDTQ - 3) : (col < 3 ? F_JDACC : F_JDTQ - 3)) + col;
+            A.st(f, gi, sum);
+            acc += sum;
+        }
+        __syncthreads();
+    }
+    // (c) finalisation and ring correction (FINALIZE_RELATIVITY, RING_CORRECTION)
+    if (lane >= 3 && lane < 9) fin[lane - 3] = acc;
+    __syncthreads();
+    if (lane == 0) finalize_with(d, A, gi, ga, v3{fin[0], fin[1], fin[2]}, v3{fin[3], fin[4], fin[5]});
+}
+
+template <bool NP>
 __global__ __launch_bounds__(64) void k_main_wpb(View d, const int2 *__restrict__ work) {
     const Acc A{d.F, d.nbp};
     const int s = work[blockIdx.x].x, il = work[blockIdx.x].y;
@@ -683,10 +777,15 @@ __global__ __launch_bounds__(64) void k_main_wpb(View d, const int2 *__restrict_
     mind = checked_max_into(A.ld(F_MIND, gi), wave_checked_max(mind));
     maxi = checked_max_into(A.ld(F_MAXI, gi), wave_checked_max(maxi));
     if (lane < 3) A.st(F_GACC + lane, gi, acc);
-    else if (lane < 6) A.st(F_DACC + lane - 3, gi, acc);
-    else if (lane < 9) A.st(F_DTQ + lane - 6, gi, acc);
     else if (lane == 9) A.st(F_MIND, gi, mind);
     else if (lane == 10) A.st(F_MAXI, gi, maxi);
+    if (NP && sm.nitem > 0) {
+        const v3 ga{__shfl(acc, 0, 64), __shfl(acc, 1, 64), __shfl(acc, 2, 64)};
+        wpb_nonpoint(d, A, sm, il, gi, acc, ga);
+        return;
+    }
+    if (lane >= 3 && lane < 6) A.st(F_DACC + lane - 3, gi, acc);
+    else if (lane >= 6 && lane < 9) A.st(F_DTQ + lane - 6, gi, acc);
     if (sm.nitem == 0 && lane == 11) {                     // FINALIZE_RELATIVITY
         A.st(F_PHI, gi, I.phi * kC2);
         A.stv(F_NACC, gi, A.ldv(F_NACCC, gi) * kC2);
@@ -1374,8 +1473,10 @@ void launch_deform(const View &d, const int4 *blk_light, int n_light, const int4
 void launch_main_pool(const View &d, const int2 *blk, int nblk, hipStream_t st) {
     if (nblk) hipLaunchKernelGGL(k_main_pool<kBS>, dim3(nblk), dim3(kBS), 0, st, d, blk);
 }
-void launch_main_wpb(const View &d, const int2 *work, int nwork, hipStream_t st) {
-    if (nwork) hipLaunchKernelGGL(k_main_wpb, dim3(nwork), dim3(64), 0, st, d, work);
+void launch_main_wpb(const View &d, const int2 *work, int nwork, bool fused_np, hipStream_t st) {
+    if (!nwork) return;
+    if (fused_np) hipLaunchKernelGGL(k_main_wpb<true>, dim3(nwork), dim3(64), 0, st, d, work);
+    else hipLaunchKernelGGL(k_main_wpb<false>, dim3(nwork), dim3(64), 0, st, d, work);
 }
 void launch_deform_wpb(const View &d, const int2 *work, int nwork, int prepare, hipStream_t st) {
     if (nwork) hipLaunchKernelGGL(k_deform_wpb, dim3(nwork), dim3(64), 0, st, d, work, prepare);

@@ -147,6 +147,7 @@ int Batch::build_plan(int mode) {
     int64_t an_rows = 0;
     pl.maxT = 1;
     pl.max_n = 1;
+    pl.np_fusable = true;
     for (int64_t s = 0; s < n_sys; ++s) {
         SysMeta &m = meta[s];
         const int64_t n = sys_n[s];
@@ -185,6 +186,7 @@ int Batch::build_plan(int mode) {
         }
         m.nitem = (int32_t)nnp;
         m.T = (int32_t)T;
+        if (n > kNpFusedMaxN_h || nnp > kNpFusedMaxItems_h) pl.np_fusable = false;
         pl.maxT = std::max<int>(pl.maxT, (int)T);
         items.insert(items.end(), sys_items.begin(), sys_items.end());
         append_blocks(blk_pool, (int)s, (int)n);
@@ -239,7 +241,7 @@ static constexpr int kWpbAutoBodies = 16384;
 
 bool Batch::use_wpb(int mode) const {
     if (variant == 1) return false;
-    if (variant == 2) return true;
+    if (variant >= 2) return true;
     return plan[mode].n_work_pool > 0 && plan[mode].n_work_pool <= kWpbAutoBodies;
 }
 
@@ -261,16 +263,17 @@ int Batch::accel(int mode, bool prepare_only) {
         }
     }
     if (prepare_only) return EPH_OK;
+    const bool fused_np = wpb && pl.np_fusable && variant != 3;
     if (pl.n_pool) {
         Scope sc(profiler, 2, stream);
-        if (wpb) launch_main_wpb(v, pl.d_work_pool, pl.n_work_pool, stream);
+        if (wpb) launch_main_wpb(v, pl.d_work_pool, pl.n_work_pool, fused_np, stream);
         else launch_main_pool(v, pl.d_blk_pool, pl.n_pool, stream);
     }
     if (pl.n_serial) {
         Scope sc(profiler, 3, stream);
         launch_main_serial(v, pl.d_blk_serial, pl.n_serial, stream);
     }
-    if (pl.nrows) {
+    if (pl.nrows && !fused_np) {
         Scope sc(profiler, 4, stream);
         launch_np(v, pl.nrows, pl.max_n, pl.maxT, pl.d_blk_recv, pl.n_recv, stream);
     }
@@ -838,7 +841,7 @@ int eph_particles_destroy(eph_particles *ps) {
 
 int eph_system_set_variant(eph_system *sys, int variant, int allow_persistent) {
     if (!sys) return fail(EPH_EINVAL, "null system");
-    if (variant < 0 || variant > 2) return fail(EPH_EINVAL, "variant must be 0 (automatic), 1 or 2");
+    if (variant < 0 || variant > 3) return fail(EPH_EINVAL, "variant must be 0 (automatic), 1, 2 or 3");
     B(sys)->variant = variant;
     B(sys)->force_generic = !allow_persistent;
     return EPH_OK;

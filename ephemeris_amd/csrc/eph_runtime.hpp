@@ -49,6 +49,7 @@ struct Plan {
     double *d_an = nullptr, *d_part = nullptr;
     int2 *d_work_pool = nullptr;     // (system, body) of every body of a pool-order system
     int n_work_pool = 0;
+    bool np_fusable = true;          // every pool system fits the fused non-point pass
 };
 
 struct Batch {

@@ -74,7 +74,9 @@ class GpuSystems:
         _lib.check(_lib.lib().eph_integrate_ex(self.h, float(dt), int(n_step), summation))
 
     def set_variant(self, variant=0, allow_persistent=True):
-        """0 automatic, 1 one thread per body, 2 one wavefront per body."""
+        """0 automatic, 1 one thread per body, 2 one wavefront per body (non-point
+        pass fused where it fits), 3 one wavefront per body with the non-point
+        pass in its own kernels."""
         _lib.check(_lib.lib().eph_system_set_variant(self.h, int(variant), int(allow_persistent)))
 
     def profile(self, enable=True):

@@ -127,7 +127,10 @@ int eph_integrate_ex(eph_system *sys, double dt, int64_t n_step, int summation);
 /* ---- tuning ------------------------------------------------------------------
  * Kernel variant of the partner-loop passes: 0 automatic, 1 one thread per
  * body (throughput; large batches), 2 one wavefront per body (latency; few
- * bodies).  allow_persistent = 0 keeps systems of <= 64 bodies off the
+ * bodies; the non-point pass runs inside the body's wavefront when every
+ * thread-pool-order system has <= 1024 bodies and <= 64 non-point items),
+ * 3 as 2 with the non-point pass in its own kernels.
+ * allow_persistent = 0 keeps systems of <= 64 bodies off the
  * one-workgroup-per-system path.  All choices give bit-identical results.
  * No counterpart in the reference (its only knob is CUDA_CORES,
  * src/integrators/RungeKutta.cu:13-18, which changes results). */
