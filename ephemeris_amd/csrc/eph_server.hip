@@ -1,12 +1,14 @@
 // Serving loop: continuous batching of ensemble and test-particle jobs over
 // state resident in HBM (eph_server_* of include/ephemeris_hip.h).
 //
-// One tick: admit queued jobs first-in first-out while their footprint fits
-// the memory budget, the device's free memory and the slots; re-form the
-// ensemble batch and the particle pool when membership changed (resident
-// state is copied device to device); launch one particle quantum on the
-// particle stream and one ensemble quantum on the batch stream so the two
-// overlap; then retire finished jobs with their results.
+// Two loops, one per job kind, each on its own stream (and, under
+// eph_server_start, its own host thread), so short particle quanta never wait
+// for a long ensemble quantum.  One tick of a loop: admit queued jobs of its
+// kind first-in first-out while their footprint fits the memory budget, the
+// device's free memory and the slots; re-form the ensemble batch or the
+// particle pool when membership changed (resident state is copied device to
+// device); advance every resident job of the kind by one quantum; retire
+// finished jobs with their results.
 //
 // Results never depend on co-residents: ensemble kernels never mix systems
 // and every job advances in calls of exactly ens_quantum steps from its own
@@ -73,12 +75,13 @@ struct Server {
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
     std::mutex mu;                       // jobs, queue, counters
-    std::mutex tick_mu;                  // one tick at a time
+    std::mutex tick_mu[2];               // one tick of each kind at a time
     std::map<int64_t, std::unique_ptr<SJob>> jobs;
     std::deque<int64_t> queue;
     int64_t next_id = 1;
     int64_t mem_used = 0, mem_peak = 0, mem_budget = 0;
     int64_t n_running = 0;
+    int64_t n_run[2] = {0, 0}, n_queued[2] = {0, 0};   // per job kind
     eph_server_stats st{};
 
     // ensemble residency
@@ -99,7 +102,7 @@ struct Server {
     bool part_dirty = false;
 
     // worker
-    std::thread worker;
+    std::thread worker[2];
     std::atomic<bool> stop_flag{false}, worker_on{false};
     std::condition_variable cv_work, cv_idle;
     std::string worker_error;
@@ -108,11 +111,15 @@ struct Server {
     int init(const eph_server_config &c, Batch *field);
     int submit(std::unique_ptr<SJob> j, int64_t *id);
     int tick(int64_t *n_active);
-    int admit(std::vector<SJob *> &new_ens, std::vector<SJob *> &new_part);
+    int tick_kind(int kind);
+    int admit(int kind, std::vector<SJob *> &fresh);
     int regroup_ensembles(const std::vector<SJob *> &new_ens);
     int rebuild_pool(const std::vector<SJob *> &new_part);
     int launch_particles_tick(std::vector<PBlock> &host_blocks, int64_t *pairs, int64_t *psteps);
-    int retire();
+    int retire_ensembles();
+    int retire_particles();
+    void mark_done(SJob *j);
+    bool has_work(int kind) { return n_queued[kind] > 0 || n_run[kind] > 0; }
     void fail_job(SJob *j, const std::string &why);
     // the job map is shared with submitting threads: look up under the lock
     SJob *find_job(int64_t id) {
@@ -126,11 +133,10 @@ struct Server {
 };
 
 Server::~Server() {
-    if (worker.joinable()) {
-        stop_flag = true;
-        cv_work.notify_all();
-        worker.join();
-    }
+    stop_flag = true;
+    cv_work.notify_all();
+    for (auto &w : worker)
+        if (w.joinable()) w.join();
     if (pst) (void)hipStreamSynchronize(pst);
     delete ens;
     if (P) (void)hipFree(P);
@@ -179,6 +185,7 @@ int Server::submit(std::unique_ptr<SJob> j, int64_t *id) {
         return fail(EPH_ENOMEM, "job needs " + std::to_string(j->bytes) + " bytes, more than the memory budget");
     if (id) *id = j->id;
     queue.push_back(j->id);
+    ++n_queued[j->kind];
     jobs[j->id] = std::move(j);
     ++st.jobs_submitted;
     cv_work.notify_all();
@@ -190,6 +197,7 @@ void Server::fail_job(SJob *j, const std::string &why) {
     if (j->status == EPH_JOB_RUNNING) {
         mem_used -= j->bytes;
         --n_running;
+        --n_run[j->kind];
     }
     j->status = EPH_JOB_FAILED;
     j->error = why;
@@ -197,59 +205,52 @@ void Server::fail_job(SJob *j, const std::string &why) {
     ++st.jobs_failed;
 }
 
-// First in, first out: stop at the first job that does not fit now.
-int Server::admit(std::vector<SJob *> &new_ens, std::vector<SJob *> &new_part) {
+// First in, first out within the kind: stop at the first job of the kind
+// that does not fit now.
+int Server::admit(int kind, std::vector<SJob *> &fresh) {
     size_t free_b = 0, total_b = 0;
     HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
     std::lock_guard<std::mutex> lk(mu);
-    int64_t systems = 0, particles = 0;
-    for (int64_t id : ens_jobs) {
+    const bool is_ens = kind == EPH_JOB_ENSEMBLE;
+    int64_t used_slots = 0;                // systems or particles resident
+    for (int64_t id : is_ens ? ens_jobs : part_jobs) {
         auto it = jobs.find(id);
-        if (it != jobs.end() && it->second->status == EPH_JOB_RUNNING) systems += (int64_t)it->second->sys_n.size();
+        if (it != jobs.end() && it->second->status == EPH_JOB_RUNNING)
+            used_slots += is_ens ? (int64_t)it->second->sys_n.size() : it->second->units;
     }
-    for (int64_t id : part_jobs) {
-        auto it = jobs.find(id);
-        if (it != jobs.end() && it->second->status == EPH_JOB_RUNNING) particles += it->second->units;
-    }
-    int64_t reserve = 0;                  // the regroup holds old and new copies at once
-    while (!queue.empty()) {
-        SJob *j = jobs[queue.front()].get();
-        const bool is_ens = j->kind == EPH_JOB_ENSEMBLE;
-        const bool slots_ok = is_ens ? (!cfg.max_systems || systems + (int64_t)j->sys_n.size() <= cfg.max_systems)
-                                     : (!cfg.max_particles || particles + j->units <= cfg.max_particles);
-        const int64_t resident = is_ens ? (int64_t)F_COUNT * 8 * (ens ? ens->nb : 0)
-                                        : (int64_t)P_COUNT_HOST * 8 * npp;
+    const int64_t max_slots = is_ens ? cfg.max_systems : cfg.max_particles;
+    const int64_t resident = is_ens ? (int64_t)F_COUNT * 8 * (ens ? ens->nb : 0) : (int64_t)P_COUNT_HOST * 8 * npp;
+    int64_t reserve = 0;                   // a re-formed batch holds old and new copies at once
+    for (auto qi = queue.begin(); qi != queue.end();) {
+        SJob *j = jobs[*qi].get();
+        if (j->kind != kind) { ++qi; continue; }
+        const int64_t want = is_ens ? (int64_t)j->sys_n.size() : j->units;
+        const bool slots_ok = !max_slots || used_slots + want <= max_slots;
         const bool mem_ok = mem_used + j->bytes <= mem_budget &&
                             (int64_t)free_b >= reserve + resident + 2 * j->bytes;
         if (!slots_ok || !mem_ok) {
-            if (n_running == 0 && !slots_ok) {
-                // could never be admitted: more systems/particles than the slots
-                queue.pop_front();
+            if (n_run[kind] == 0 && fresh.empty()) {     // could never be admitted
+                qi = queue.erase(qi);
+                --n_queued[kind];
                 j->status = EPH_JOB_FAILED;
-                j->error = "job exceeds the server's slots";
-                j->t_done = clock_s(t0);
-                ++st.jobs_failed;
-                continue;
-            }
-            if (n_running == 0 && !mem_ok) {
-                queue.pop_front();
-                j->status = EPH_JOB_FAILED;
-                j->error = "job does not fit in free device memory";
+                j->error = slots_ok ? "job does not fit in free device memory" : "job exceeds the server's slots";
                 j->t_done = clock_s(t0);
                 ++st.jobs_failed;
                 continue;
             }
             break;
         }
-        queue.pop_front();
+        qi = queue.erase(qi);
+        --n_queued[kind];
         j->status = EPH_JOB_RUNNING;
         j->t_admit = clock_s(t0);
         mem_used += j->bytes;
         mem_peak = std::max(mem_peak, mem_used);
         reserve += 2 * j->bytes;
         ++n_running;
-        if (is_ens) { new_ens.push_back(j); systems += (int64_t)j->sys_n.size(); }
-        else { new_part.push_back(j); particles += j->units; }
+        ++n_run[kind];
+        used_slots += want;
+        fresh.push_back(j);
     }
     return EPH_OK;
 }
@@ -378,48 +379,54 @@ int Server::launch_particles_tick(std::vector<PBlock> &hb, int64_t *pairs, int64
     return EPH_OK;
 }
 
+void Server::mark_done(SJob *j) {
+    std::lock_guard<std::mutex> lk(mu);
+    j->status = EPH_JOB_DONE;
+    j->t_done = clock_s(t0);
+    mem_used -= j->bytes;
+    --n_running;
+    --n_run[j->kind];
+    ++st.jobs_done;
+}
+
 // Move finished jobs' results to the host and free their residency.
-int Server::retire() {
-    std::vector<SJob *> done_ens, done_part;
+int Server::retire_ensembles() {
+    std::vector<SJob *> done;
     for (int64_t id : ens_jobs) {
         SJob *j = find_job(id);
-        if (j->status == EPH_JOB_RUNNING && j->steps_done >= j->steps_total) done_ens.push_back(j);
+        if (j->status == EPH_JOB_RUNNING && j->steps_done >= j->steps_total) done.push_back(j);
     }
+    if (done.empty()) return EPH_OK;
+    std::vector<eph_mass> all(ens->nb);
+    int rc = ens->download(all.data());
+    if (rc) return rc;
+    for (SJob *j : done) {
+        int64_t k = 0, s0 = -1;
+        for (int64_t s = 0; s < ens->n_sys && s0 < 0; ++s)
+            if (ens_owner[s] == j->id) s0 = s;
+        bool ok = true;
+        for (int64_t s = s0; s < s0 + (int64_t)j->sys_n.size(); ++s) {
+            for (int64_t g = ens->sys_off[s]; g < ens->sys_off[s + 1]; ++g, ++k) {
+                eph_mass m = all[g];
+                m.gp = j->masses[k].gp;               // model indices local to the job
+                m.ring = j->masses[k].ring;
+                ok = ok && finite_state(m);
+                j->masses[k] = m;
+            }
+            j->t_eph[2 * (s - s0)] = ens->t_eph[s].hi;
+            j->t_eph[2 * (s - s0) + 1] = ens->t_eph[s].lo;
+        }
+        if (!ok) fail_job(j, "state became non-finite");
+        else mark_done(j);
+    }
+    ens_dirty = true;
+    return EPH_OK;
+}
+
+int Server::retire_particles() {
     for (int64_t id : part_jobs) {
         SJob *j = find_job(id);
-        if (j->status == EPH_JOB_RUNNING && j->steps_done >= j->steps_total) done_part.push_back(j);
-    }
-    if (!done_ens.empty()) {
-        std::vector<eph_mass> all(ens->nb);
-        int rc = ens->download(all.data());
-        if (rc) return rc;
-        for (SJob *j : done_ens) {
-            int64_t k = 0, s0 = -1;
-            for (int64_t s = 0; s < ens->n_sys; ++s)
-                if (ens_owner[s] == j->id) { if (s0 < 0) s0 = s; }
-            bool ok = true;
-            for (int64_t s = s0; s < s0 + (int64_t)j->sys_n.size(); ++s) {
-                for (int64_t g = ens->sys_off[s]; g < ens->sys_off[s + 1]; ++g, ++k) {
-                    eph_mass m = all[g];
-                    m.gp = j->masses[k].gp;               // model indices local to the job
-                    m.ring = j->masses[k].ring;
-                    ok = ok && finite_state(m);
-                    j->masses[k] = m;
-                }
-                j->t_eph[2 * (s - s0)] = ens->t_eph[s].hi;
-                j->t_eph[2 * (s - s0) + 1] = ens->t_eph[s].lo;
-            }
-            if (!ok) { fail_job(j, "state became non-finite"); continue; }
-            std::lock_guard<std::mutex> lk(mu);
-            j->status = EPH_JOB_DONE;
-            j->t_done = clock_s(t0);
-            mem_used -= j->bytes;
-            --n_running;
-            ++st.jobs_done;
-        }
-        ens_dirty = true;
-    }
-    for (SJob *j : done_part) {
+        if (j->status != EPH_JOB_RUNNING || j->steps_done < j->steps_total) continue;
         const int64_t n = j->units;
         std::vector<double> h((size_t)P_COUNT_HOST * n);
         HIP_CHECK(hipMemcpy2D(h.data(), sizeof(double) * n, P + j->col, sizeof(double) * npp, sizeof(double) * n,
@@ -438,69 +445,81 @@ int Server::retire() {
             j->ext[2 * i + 1] = h[(size_t)16 * n + i];
         }
         part_dirty = true;
-        if (!ok) { fail_job(j, "particle state became non-finite"); continue; }
-        std::lock_guard<std::mutex> lk(mu);
-        j->status = EPH_JOB_DONE;
-        j->t_done = clock_s(t0);
-        mem_used -= j->bytes;
-        --n_running;
-        ++st.jobs_done;
+        if (!ok) fail_job(j, "particle state became non-finite");
+        else mark_done(j);
     }
     return EPH_OK;
 }
 
-int Server::tick(int64_t *n_active) {
-    std::lock_guard<std::mutex> tk(tick_mu);
+// One tick of the loop of `kind`.
+int Server::tick_kind(int kind) {
+    std::lock_guard<std::mutex> tk(tick_mu[kind]);
     HIP_CHECK(hipSetDevice(device));
     const double t_begin = clock_s(t0);
-    std::vector<SJob *> new_ens, new_part;
-    int rc = admit(new_ens, new_part);
+    std::vector<SJob *> fresh;
+    int rc = admit(kind, fresh);
     if (rc) return rc;
-    if (!new_ens.empty() || ens_dirty) {
-        if ((rc = regroup_ensembles(new_ens))) {
-            const std::string why = std::string("ensemble admission failed: ") + eph_last_error();
-            for (SJob *j : new_ens) fail_job(j, why);
-            return rc;
+    double pairs = 0, sys_steps = 0, psteps = 0;
+    if (kind == EPH_JOB_ENSEMBLE) {
+        if (!fresh.empty() || ens_dirty) {
+            if ((rc = regroup_ensembles(fresh))) {
+                const std::string why = std::string("ensemble admission failed: ") + eph_last_error();
+                for (SJob *j : fresh) fail_job(j, why);
+                return rc;
+            }
+            ens_dirty = false;
         }
-        ens_dirty = false;
-    }
-    if (!new_part.empty() || part_dirty) {
-        if ((rc = rebuild_pool(new_part))) {
-            const std::string why = std::string("particle admission failed: ") + eph_last_error();
-            for (SJob *j : new_part) fail_job(j, why);
-            return rc;
-        }
-        part_dirty = false;
-    }
-    // one quantum of every resident job; the particle launch overlaps the ensemble
-    std::vector<PBlock> hb;
-    int64_t ppairs = 0, psteps = 0;
-    if ((rc = launch_particles_tick(hb, &ppairs, &psteps))) return rc;
-    double epairs = 0, esteps = 0;
-    if (ens) {
+        if (!ens) return EPH_OK;
         if ((rc = ens->integrate(cfg.ens_dt, cfg.ens_quantum, 0))) return rc;
-        for (int64_t n : ens->sys_n) epairs += (double)n * (double)(n - 1);
-        epairs *= 25.0 * (double)cfg.ens_quantum;
-        esteps = (double)cfg.ens_quantum * (double)ens->n_sys;
-    }
-    HIP_CHECK(hipStreamSynchronize(pst));
-    {
-        std::lock_guard<std::mutex> lk(mu);
-        for (int64_t id : ens_jobs) jobs.at(id)->steps_done += cfg.ens_quantum;
-        for (int64_t id : part_jobs) {
-            SJob *j = jobs.at(id).get();
-            j->steps_done += std::min(cfg.part_quantum, j->steps_total - j->steps_done);
+        for (int64_t n : ens->sys_n) pairs += (double)n * (double)(n - 1);
+        pairs *= 25.0 * (double)cfg.ens_quantum;
+        sys_steps = (double)cfg.ens_quantum * (double)ens->n_sys;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            for (int64_t id : ens_jobs) jobs.at(id)->steps_done += cfg.ens_quantum;
         }
-        st.pair_interactions += epairs + (double)ppairs;
-        st.rk12_system_steps += esteps;
-        st.particle_steps += (double)psteps;
+        if ((rc = retire_ensembles())) return rc;
+    } else {
+        if (!fresh.empty() || part_dirty) {
+            if ((rc = rebuild_pool(fresh))) {
+                const std::string why = std::string("particle admission failed: ") + eph_last_error();
+                for (SJob *j : fresh) fail_job(j, why);
+                return rc;
+            }
+            part_dirty = false;
+        }
+        if (!npp) return EPH_OK;
+        std::vector<PBlock> hb;
+        int64_t pp = 0, ps = 0;
+        if ((rc = launch_particles_tick(hb, &pp, &ps))) return rc;
+        HIP_CHECK(hipStreamSynchronize(pst));
+        pairs = (double)pp;
+        psteps = (double)ps;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            for (int64_t id : part_jobs) {
+                SJob *j = jobs.at(id).get();
+                j->steps_done += std::min(cfg.part_quantum, j->steps_total - j->steps_done);
+            }
+        }
+        if ((rc = retire_particles())) return rc;
     }
-    if ((rc = retire())) return rc;
     std::lock_guard<std::mutex> lk(mu);
+    st.pair_interactions += pairs;
+    st.rk12_system_steps += sys_steps;
+    st.particle_steps += psteps;
     ++st.ticks;
     st.busy_s += clock_s(t0) - t_begin;
-    if (n_active) *n_active = n_running;
     if (n_running == 0 && queue.empty()) cv_idle.notify_all();
+    return EPH_OK;
+}
+
+int Server::tick(int64_t *n_active) {
+    int rc = tick_kind(EPH_JOB_ENSEMBLE);
+    if (rc) return rc;
+    if ((rc = tick_kind(EPH_JOB_PARTICLES))) return rc;
+    std::lock_guard<std::mutex> lk(mu);
+    if (n_active) *n_active = n_running;
     return EPH_OK;
 }
 
@@ -603,30 +622,32 @@ int eph_server_start(eph_server *srv) {
     if (s->worker_on) return EPH_OK;
     s->stop_flag = false;
     s->worker_on = true;
-    s->worker = std::thread([s] {
-        if (hipSetDevice(s->device) != hipSuccess) {
-            std::lock_guard<std::mutex> lk(s->mu);
-            s->worker_error = "hipSetDevice failed in the worker";
-            s->cv_idle.notify_all();
-            return;
-        }
-        while (!s->stop_flag) {
-            {
-                std::unique_lock<std::mutex> lk(s->mu);
-                s->cv_work.wait_for(lk, std::chrono::milliseconds(50),
-                                    [s] { return s->stop_flag || !s->queue.empty() || s->n_running > 0; });
-                if (s->stop_flag) break;
-                if (s->queue.empty() && s->n_running == 0) continue;
-            }
-            int64_t n = 0;
-            if (s->tick(&n)) {
+    for (int kind = 0; kind < 2; ++kind)
+        s->worker[kind] = std::thread([s, kind] {
+            if (hipSetDevice(s->device) != hipSuccess) {
                 std::lock_guard<std::mutex> lk(s->mu);
-                s->worker_error = eph_last_error();
+                s->worker_error = "hipSetDevice failed in a worker";
                 s->cv_idle.notify_all();
-                break;
+                return;
             }
-        }
-    });
+            while (!s->stop_flag) {
+                {
+                    std::unique_lock<std::mutex> lk(s->mu);
+                    s->cv_work.wait_for(lk, std::chrono::milliseconds(50),
+                                        [s, kind] { return s->stop_flag || s->has_work(kind); });
+                    if (s->stop_flag) break;
+                    if (!s->has_work(kind)) continue;
+                }
+                if (s->tick_kind(kind)) {
+                    std::lock_guard<std::mutex> lk(s->mu);
+                    s->worker_error = eph_last_error();
+                    s->stop_flag = true;
+                    s->cv_idle.notify_all();
+                    s->cv_work.notify_all();
+                    break;
+                }
+            }
+        });
     return EPH_OK;
 }
 
@@ -648,11 +669,10 @@ int eph_server_wait(eph_server *srv, double timeout_s) {
 int eph_server_stop(eph_server *srv) {
     if (!srv) return fail(EPH_EINVAL, "null server");
     Server *s = S(srv);
-    if (s->worker.joinable()) {
-        s->stop_flag = true;
-        s->cv_work.notify_all();
-        s->worker.join();
-    }
+    s->stop_flag = true;
+    s->cv_work.notify_all();
+    for (auto &w : s->worker)
+        if (w.joinable()) w.join();
     s->worker_on = false;
     return EPH_OK;
 }
