@@ -628,31 +628,35 @@ __device__ __forceinline__ double wave_checked_max(double x) {
     return x;
 }
 
-constexpr int kNpFusedMaxN = 1024;    // systems up to this size take the fused non-point pass
+constexpr int kNpFusedMaxN = 512;     // systems up to this size take the fused non-point pass
 constexpr int kNpFusedMaxItems = 64;
 
-// Non-point pass of body i inside its own wavefront (k_np_field, k_np_item
-// and k_np_recv for one receiver): receiver terms in round order, then the
-// sums of i's own items over partner slices, then finalisation.  `acc` of
-// lanes 3-5 holds daccel, of lanes 6-8 dtorque; `ga` is gaccel.
+// Non-point pass of body i inside its own wavefront, after k_np_field has
+// left every item's field at every partner in d.an (k_np_item and k_np_recv
+// for one receiver): receiver terms in round order, then the sums of i's own
+// items over partner slices, then finalisation.  `acc` of lanes 3-5 holds
+// daccel, of lanes 6-8 dtorque; `ga` is gaccel.
+// pq: LDS scratch of 6 * n doubles (the pair-term tiles, no longer needed)
 __device__ __forceinline__ void wpb_nonpoint(const View &d, const Acc &A, const SysMeta &sm, int il, int gi,
-                                             double &acc, const v3 &ga) {
-    const int lane = (int)threadIdx.x;
+                                             double &acc, const v3 &ga, double *pq) {
+    const int lane = (int)threadIdx.x, nthreads = (int)blockDim.x;
     const int n = sm.n, nit = sm.nitem, T = sm.T;
-    __shared__ double rf[kNpFusedMaxItems * 3];
-    __shared__ double an[3 * kNpFusedMaxN];
+    __shared__ double rf[kNpFusedMaxItems * 3];        // GM_p * field of item q at r_i (0 for i's own items)
     __shared__ double part[kNpFusedMaxItems * 6];
     __shared__ double fin[6];
     __shared__ int own_q[2];
-    const ddv3 ri = A.lddv(F_R, gi);
     if (lane < 2) own_q[lane] = -1;
     __syncthreads();
-    // (a) field of every item at r_i (mass.cpp:283-297, receiver side)
+    // (a) receiver side (mass.cpp:283-297): GM_p * an of every item at i
     if (lane < nit) {
         const NPItem it = d.items[sm.item0 + lane];
         v3 f{0, 0, 0};
-        if (it.body != gi) f = nonpoint_field(d, A, it.body, it.kind, it.model, diff(ri, A.lddv(F_R, it.body)));
-        else own_q[it.kind] = lane;
+        if (it.body != gi) {
+            const double *a = d.an + 3 * ((int64_t)it.an_off + il);
+            f = A.ld(F_GM, it.body) * v3{a[0], a[1], a[2]};
+        } else {
+            own_q[it.kind] = lane;
+        }
         rf[3 * lane] = f.x; rf[3 * lane + 1] = f.y; rf[3 * lane + 2] = f.z;
     }
     __syncthreads();
@@ -664,24 +668,28 @@ __device__ __forceinline__ void wpb_nonpoint(const View &d, const Acc &A, const 
             int cc = sj - k;
             if (cc < 0) cc += T;
             const int first = cc * nit / T, last = (cc + 1) * nit / T;
-            for (int q = last - 1; q >= first; --q) {
-                const int pb = d.items[sm.item0 + q].body;
-                if (pb == gi) continue;
-                acc += A.ld(F_GM, pb) * rf[3 * q + c];
-            }
+            for (int q = last - 1; q >= first; --q)
+                if (q != own_q[0] && q != own_q[1]) acc += rf[3 * q + c];
         }
     }
     // (b) own items: geopotential (idaccel) first, then ring (jdaccel), mass.cpp:298-307
+    const ddv3 ri = A.lddv(F_R, gi);
     for (int kind = 0; kind < 2; ++kind) {
         const int q = own_q[kind];
         if (q < 0) continue;
         const NPItem it = d.items[sm.item0 + q];
-        for (int t0 = 0; t0 < n; t0 += 64) {
-            const int jl = t0 + lane;
-            if (jl >= n) break;
-            v3 f{0, 0, 0};
-            if (jl != il) f = nonpoint_field(d, A, gi, kind, it.model, diff(A.lddv(F_R, sm.off + jl), ri));
-            an[3 * jl] = f.x; an[3 * jl + 1] = f.y; an[3 * jl + 2] = f.z;
+        for (int jl = lane; jl < n; jl += nthreads) {     // pq[j]: GM_j * field at j, GM_j * r x field
+            v3 ta{0, 0, 0}, tt{0, 0, 0};
+            if (jl != il) {
+                const int gj = sm.off + jl;
+                const double *a = d.an + 3 * ((int64_t)it.an_off + jl);
+                const v3 an{a[0], a[1], a[2]};
+                const double GMj = A.ld(F_GM, gj);
+                ta = GMj * an;
+                tt = GMj * cross(diff(A.lddv(F_R, gj), ri), an);
+            }
+            double *o = pq + 6 * jl;
+            o[0] = ta.x; o[1] = ta.y; o[2] = ta.z; o[3] = tt.x; o[4] = tt.y; o[5] = tt.z;
         }
         __syncthreads();
         if (lane < T) {                                    // round `lane` visits slice (chain + lane) mod T
@@ -689,17 +697,13 @@ __device__ __forceinline__ void wpb_nonpoint(const View &d, const Acc &A, const 
             if (kk >= T) kk -= T;
             const int j0 = kk * n / T, j1 = (kk + 1) * n / T;
             v3 td{0, 0, 0}, tq{0, 0, 0};
-            for (int jl = j0; jl < j1; ++jl) {
-                if (jl == il) continue;
-                const int gj = sm.off + jl;
-                const v3 r = diff(A.lddv(F_R, gj), ri);
-                const v3 a{an[3 * jl], an[3 * jl + 1], an[3 * jl + 2]};
-                const double GMj = A.ld(F_GM, gj);
-                td -= GMj * a;
-                tq -= GMj * cross(r, a);
+            for (int jl = j0; jl < j1; ++jl) {             // the self pair holds +0: an exact identity
+                const double *o = pq + 6 * jl;
+                td -= v3{o[0], o[1], o[2]};
+                tq -= v3{o[3], o[4], o[5]};
             }
-            double *o = part + 6 * lane;
-            o[0] = td.x; o[1] = td.y; o[2] = td.z; o[3] = tq.x; o[4] = tq.y; o[5] = tq.z;
+            double *w = part + 6 * lane;
+            w[0] = td.x; w[1] = td.y; w[2] = td.z; w[3] = tq.x; w[4] = tq.y; w[5] = tq.z;
         }
         __syncthreads();
         if (lane >= 3 && lane < 9) {
@@ -718,23 +722,26 @@ __device__ __forceinline__ void wpb_nonpoint(const View &d, const Acc &A, const 
     if (lane == 0) finalize_with(d, A, gi, ga, v3{fin[0], fin[1], fin[2]}, v3{fin[3], fin[4], fin[5]});
 }
 
-template <bool NP>
-__global__ __launch_bounds__(64) void k_main_wpb(View d, const int2 *__restrict__ work) {
+template <bool NP, int W>
+__global__ __launch_bounds__(64 * W) void k_main_wpb(View d, const int2 *__restrict__ work) {
+    static_assert(!NP || W * kWpbRows * kWpbPitch >= 6 * kNpFusedMaxN, "term tiles double as non-point scratch");
     const Acc A{d.F, d.nbp};
     const int s = work[blockIdx.x].x, il = work[blockIdx.x].y;
     const SysMeta sm = d.sys[s];
     const int n = sm.n, off = sm.off, gi = off + il;
-    const int lane = (int)threadIdx.x;
-    __shared__ double term[kWpbRows * kWpbPitch];
+    const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
+    __shared__ double term[W * kWpbRows * kWpbPitch];
+    __shared__ double wmax[2 * W];
     const IMain I = load_imain(A, gi);
-    term[kWpbZero * kWpbPitch + lane] = 0.0;
+    double *my = term + wv * kWpbRows * kWpbPitch;
+    my[kWpbZero * kWpbPitch + lane] = 0.0;
     int row[kWpbSteps];
 #pragma unroll
     for (int k = 0; k < kWpbSteps; ++k) row[k] = wpb_main_row(lane, k) * kWpbPitch;
     double acc = 0;
     double mind = 0, maxi = 0;                             // lane-local maxima (>= +0: identities)
-    for (int t0 = 0; t0 < n; t0 += 64) {
-        const int jl = t0 + lane;
+    for (int t0 = 0; t0 < n; t0 += 64 * W) {
+        const int jl = t0 + 64 * wv + lane;
         double t[kWpbZero];
         if (jl < n && jl != il) {
             JMain T;
@@ -763,27 +770,46 @@ __global__ __launch_bounds__(64) void k_main_wpb(View d, const int2 *__restrict_
 #pragma unroll
             for (int q = 0; q < kWpbZero; ++q) t[q] = 0.0;
         }
+        if (t0 + 64 * wv < n) {
 #pragma unroll
-        for (int q = 0; q < kWpbZero; ++q) term[q * kWpbPitch + lane] = t[q];
+            for (int q = 0; q < kWpbZero; ++q) my[q * kWpbPitch + lane] = t[q];
+        }
         __syncthreads();
-        if (lane < 9) {
+        if (wv == 0 && lane < 9) {
+            const int nw = min(W, (n - t0 + 63) / 64);
+            for (int u = 0; u < nw; ++u) {
+                const double *tile = term + u * kWpbRows * kWpbPitch;
 #pragma unroll 8
-            for (int q = 0; q < 64; ++q)
+                for (int q = 0; q < 64; ++q)
 #pragma unroll
-                for (int k = 0; k < kWpbSteps; ++k) acc += term[row[k] + q];
+                    for (int k = 0; k < kWpbSteps; ++k) acc += tile[row[k] + q];
+            }
         }
         __syncthreads();
     }
-    mind = checked_max_into(A.ld(F_MIND, gi), wave_checked_max(mind));
-    maxi = checked_max_into(A.ld(F_MAXI, gi), wave_checked_max(maxi));
-    if (lane < 3) A.st(F_GACC + lane, gi, acc);
-    else if (lane == 9) A.st(F_MIND, gi, mind);
-    else if (lane == 10) A.st(F_MAXI, gi, maxi);
+    mind = wave_checked_max(mind);
+    maxi = wave_checked_max(maxi);
+    if (lane == 0) { wmax[wv] = mind; wmax[W + wv] = maxi; }
+    __syncthreads();
+    if (wv == 0) {
+        for (int u = 1; u < W; ++u) {
+            mind = checked_max_into(mind, wmax[u]);
+            maxi = checked_max_into(maxi, wmax[W + u]);
+        }
+        mind = checked_max_into(A.ld(F_MIND, gi), mind);
+        maxi = checked_max_into(A.ld(F_MAXI, gi), maxi);
+        if (lane < 3) A.st(F_GACC + lane, gi, acc);
+        else if (lane == 9) A.st(F_MIND, gi, mind);
+        else if (lane == 10) A.st(F_MAXI, gi, maxi);
+    }
     if (NP && sm.nitem > 0) {
-        const v3 ga{__shfl(acc, 0, 64), __shfl(acc, 1, 64), __shfl(acc, 2, 64)};
-        wpb_nonpoint(d, A, sm, il, gi, acc, ga);
+        __shared__ double gsh[3];
+        if (threadIdx.x < 3) gsh[threadIdx.x] = acc;
+        __syncthreads();
+        wpb_nonpoint(d, A, sm, il, gi, acc, v3{gsh[0], gsh[1], gsh[2]}, term);
         return;
     }
+    if (wv != 0) return;
     if (lane >= 3 && lane < 6) A.st(F_DACC + lane - 3, gi, acc);
     else if (lane >= 6 && lane < 9) A.st(F_DTQ + lane - 6, gi, acc);
     if (sm.nitem == 0 && lane == 11) {                     // FINALIZE_RELATIVITY
@@ -797,15 +823,19 @@ __global__ __launch_bounds__(64) void k_main_wpb(View d, const int2 *__restrict_
 // subtracted terms (phi, tidal matrix) are stored negated and the self pair
 // contributes +0, as in k_main_wpb.
 constexpr int kWpbDefPitch = 65;
-__global__ __launch_bounds__(64) void k_deform_wpb(View d, const int2 *__restrict__ work, int prepare) {
+// W waves per body: in each round wave w evaluates partner chunk
+// (round * W + w); wave 0 then adds the W chunks in partner order.
+template <int W>
+__global__ __launch_bounds__(64 * W) void k_deform_wpb(View d, const int2 *__restrict__ work, int prepare) {
     const Acc A{d.F, d.nbp};
     const int s = work[blockIdx.x].x, il = work[blockIdx.x].y;
     const SysMeta sm = d.sys[s];
     const int n = sm.n, off = sm.off, gi = off + il;
-    const int lane = (int)threadIdx.x;
-    __shared__ double term[13 * kWpbDefPitch];
+    const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
+    __shared__ double term[W][13 * kWpbDefPitch];
     __shared__ double sums[13];
     __shared__ int again_s;
+    __shared__ double wsh[3];
     const ddv3 ri = A.lddv(F_R, gi), vi = A.lddv(F_V, gi);
     const double sR2 = A.ld(F_SR2, gi), k2 = A.ld(F_K2, gi), td = A.ld(F_TD, gi);
     const bool damped = k2 != 0 && td != 0, ndelay = k2 != 0 && td == 0;
@@ -816,8 +846,8 @@ __global__ __launch_bounds__(64) void k_deform_wpb(View d, const int2 *__restric
     int iters_left = 4;
     for (;;) {
         double acc = 0;
-        for (int t0 = 0; t0 < n; t0 += 64) {
-            const int jl = t0 + lane;
+        for (int t0 = 0; t0 < n; t0 += 64 * W) {
+            const int jl = t0 + 64 * wv + lane;
             double t[13];
 #pragma unroll
             for (int q = 0; q < 13; ++q) t[q] = 0.0;
@@ -856,19 +886,24 @@ __global__ __launch_bounds__(64) void k_deform_wpb(View d, const int2 *__restric
                     for (int q = 0; q < 9; ++q) t[4 + q] = -tp[q];
                 }
             }
+            if (t0 + 64 * wv < n) {
 #pragma unroll
-            for (int q = 0; q < 13; ++q) term[q * kWpbDefPitch + lane] = t[q];
+                for (int q = 0; q < 13; ++q) term[wv][q * kWpbDefPitch + lane] = t[q];
+            }
             __syncthreads();
-            if (lane < nacc_chains) {
-                const double *row = term + lane * kWpbDefPitch;
+            if (wv == 0 && lane < nacc_chains) {
+                const int nw = min(W, (n - t0 + 63) / 64);
+                for (int u = 0; u < nw; ++u) {
+                    const double *row = term[u] + lane * kWpbDefPitch;
 #pragma unroll 16
-                for (int q = 0; q < 64; ++q) acc += row[q];
+                    for (int q = 0; q < 64; ++q) acc += row[q];
+                }
             }
             __syncthreads();
         }
-        if (lane < 13) sums[lane] = acc;
+        if (wv == 0 && lane < 13) sums[lane] = acc;
         __syncthreads();
-        if (lane == 0) {
+        if (threadIdx.x == 0) {
             const m3 Csum{v3{sums[4], sums[5], sums[6]}, v3{sums[7], sums[8], sums[9]}, v3{sums[10], sums[11], sums[12]}};
             if (iters_left == 4) H.load(A, gi);
             const ddv3 GL = A.lddv(F_GL, gi);
@@ -888,10 +923,11 @@ __global__ __launch_bounds__(64) void k_deform_wpb(View d, const int2 *__restric
             again_s = brk ? 0 : 1;
             if (brk) deform_write(A, gi, prepare, vi, sums[0], v3{sums[1], sums[2], sums[3]}, C, GI, w);
             else A.stv(F_W, gi, w);
+            wsh[0] = w.x; wsh[1] = w.y; wsh[2] = w.z;
         }
         __syncthreads();
         if (!again_s) break;
-        w = A.ldv(F_W, gi);
+        w = v3{wsh[0], wsh[1], wsh[2]};
         __syncthreads();
     }
 }
@@ -1473,16 +1509,25 @@ void launch_deform(const View &d, const int4 *blk_light, int n_light, const int4
 void launch_main_pool(const View &d, const int2 *blk, int nblk, hipStream_t st) {
     if (nblk) hipLaunchKernelGGL(k_main_pool<kBS>, dim3(nblk), dim3(kBS), 0, st, d, blk);
 }
-void launch_main_wpb(const View &d, const int2 *work, int nwork, bool fused_np, hipStream_t st) {
+void launch_main_wpb(const View &d, const int2 *work, int nwork, int max_n, bool fused_np, hipStream_t st) {
     if (!nwork) return;
-    if (fused_np) hipLaunchKernelGGL(k_main_wpb<true>, dim3(nwork), dim3(64), 0, st, d, work);
-    else hipLaunchKernelGGL(k_main_wpb<false>, dim3(nwork), dim3(64), 0, st, d, work);
+    if (fused_np) hipLaunchKernelGGL((k_main_wpb<true, 2>), dim3(nwork), dim3(128), 0, st, d, work);
+    else if (max_n > 64) hipLaunchKernelGGL((k_main_wpb<false, 2>), dim3(nwork), dim3(128), 0, st, d, work);
+    else hipLaunchKernelGGL((k_main_wpb<false, 1>), dim3(nwork), dim3(64), 0, st, d, work);
 }
-void launch_deform_wpb(const View &d, const int2 *work, int nwork, int prepare, hipStream_t st) {
-    if (nwork) hipLaunchKernelGGL(k_deform_wpb, dim3(nwork), dim3(64), 0, st, d, work, prepare);
+void launch_deform_wpb(const View &d, const int2 *work, int nwork, int max_n, int prepare, hipStream_t st) {
+    if (!nwork) return;
+    const int chunks = (max_n + 63) / 64;
+    if (chunks >= 8) hipLaunchKernelGGL(k_deform_wpb<8>, dim3(nwork), dim3(512), 0, st, d, work, prepare);
+    else if (chunks >= 4) hipLaunchKernelGGL(k_deform_wpb<4>, dim3(nwork), dim3(256), 0, st, d, work, prepare);
+    else if (chunks >= 2) hipLaunchKernelGGL(k_deform_wpb<2>, dim3(nwork), dim3(128), 0, st, d, work, prepare);
+    else hipLaunchKernelGGL(k_deform_wpb<1>, dim3(nwork), dim3(64), 0, st, d, work, prepare);
 }
 void launch_main_serial(const View &d, const int2 *blk, int nblk, hipStream_t st) {
     if (nblk) hipLaunchKernelGGL(k_main_serial<kBS>, dim3(nblk), dim3(kBS), 0, st, d, blk);
+}
+void launch_np_field(const View &d, int nrows, int max_n, hipStream_t st) {
+    if (nrows) hipLaunchKernelGGL(k_np_field, dim3((max_n + 63) / 64, nrows), dim3(64), 0, st, d, nrows);
 }
 void launch_np(const View &d, int nrows, int max_n, int maxT, const int2 *recv_blk, int nrecv, hipStream_t st) {
     if (!nrows) return;

@@ -58,11 +58,15 @@ void launch_deform(const View &d, const int4 *blk_light, int n_light, const int4
 void launch_main_pool(const View &d, const int2 *blk, int nblk, hipStream_t st);
 void launch_main_serial(const View &d, const int2 *blk, int nblk, hipStream_t st);
 // latency path, one wavefront per body; work = {system, body index in system}
-// fused_np: the non-point pass runs inside each body's wavefront (every
-// thread-pool-order system has at most 1024 bodies and 64 non-point items)
-void launch_main_wpb(const View &d, const int2 *work, int nwork, bool fused_np, hipStream_t st);
-constexpr int kNpFusedMaxN_h = 1024, kNpFusedMaxItems_h = 64;
-void launch_deform_wpb(const View &d, const int2 *work, int nwork, int prepare, hipStream_t st);
+// fused_np: the receiver and own-item sums of the non-point pass run inside
+// each body's wavefront, after launch_np_field (every thread-pool-order
+// system has at most 512 bodies and 64 non-point items)
+void launch_main_wpb(const View &d, const int2 *work, int nwork, int max_n, bool fused_np, hipStream_t st);
+void launch_np_field(const View &d, int nrows, int max_n, hipStream_t st);
+constexpr int kNpFusedMaxN_h = 512, kNpFusedMaxItems_h = 64;
+// one workgroup of up to 8 wavefronts per body (partner chunks evaluated in
+// parallel, summed in order); max_n = largest system of the batch
+void launch_deform_wpb(const View &d, const int2 *work, int nwork, int max_n, int prepare, hipStream_t st);
 void launch_np(const View &d, int nrows, int max_n, int maxT, const int2 *recv_blk, int nrecv, hipStream_t st);
 void launch_stage(const View &d, int k, double dt, double dt_k, double tadd, hipStream_t st);
 void launch_extrema(const View &d, int invert, hipStream_t st);

@@ -252,21 +252,25 @@ int Batch::accel(int mode, bool prepare_only) {
     {
         Scope sc(profiler, 1, stream);
         if (wpb) {
-            launch_deform_wpb(v, d_work_all, n_work_all, prepare_only ? 0 : 1, stream);
+            launch_deform_wpb(v, d_work_all, n_work_all, max_sys_n, prepare_only ? 0 : 1, stream);
         } else if (variant == 1) {
             launch_deform(v, d_blk_light, n_blk_light, d_blk_damped, n_blk_damped, prepare_only ? 0 : 1, stream);
         } else {
             // constant-time-lag bodies re-sum their partners once per spin pass:
             // one wavefront each, the rest one thread each
-            launch_deform_wpb(v, d_work_damped, n_work_damped, prepare_only ? 0 : 1, stream);
+            launch_deform_wpb(v, d_work_damped, n_work_damped, max_sys_n, prepare_only ? 0 : 1, stream);
             launch_deform(v, d_blk_light, n_blk_light, nullptr, 0, prepare_only ? 0 : 1, stream);
         }
     }
     if (prepare_only) return EPH_OK;
     const bool fused_np = wpb && pl.np_fusable && variant != 3;
+    if (fused_np && pl.nrows) {
+        Scope sc(profiler, 4, stream);
+        launch_np_field(v, pl.nrows, pl.max_n, stream);
+    }
     if (pl.n_pool) {
         Scope sc(profiler, 2, stream);
-        if (wpb) launch_main_wpb(v, pl.d_work_pool, pl.n_work_pool, fused_np, stream);
+        if (wpb) launch_main_wpb(v, pl.d_work_pool, pl.n_work_pool, pl.max_n, fused_np, stream);
         else launch_main_pool(v, pl.d_blk_pool, pl.n_pool, stream);
     }
     if (pl.n_serial) {
@@ -469,6 +473,7 @@ int create_batch(int64_t n_sys, const int64_t *sys_n, const eph_mass *masses,
         b->sys_off[s + 1] = b->sys_off[s] + sys_n[s];
     }
     b->nb = b->sys_off[n_sys];
+    b->max_sys_n = *std::max_element(b->sys_n.begin(), b->sys_n.end());
     b->nbp = (b->nb + 63) / 64 * 64;
     b->t_eph.resize(n_sys);
     for (int64_t s = 0; s < n_sys; ++s) b->t_eph[s] = t_eph ? dd{t_eph[2 * s], t_eph[2 * s + 1]} : dd{0, 0};

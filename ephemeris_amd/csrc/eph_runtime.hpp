@@ -56,6 +56,7 @@ struct Batch {
     int device = 0;
     hipStream_t stream = nullptr;
     int64_t n_sys = 0, nb = 0, nbp = 0;
+    int max_sys_n = 0;
     std::vector<int64_t> sys_n, sys_off;
     std::vector<eph_geopotential> gps;
     std::vector<eph_ring> rings;
