@@ -35,6 +35,7 @@ SIGNATURES = {
     "eph_profile_read": (I, [P, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.POINTER(C.c_int64), I]),
     "eph_profile_reset": (I, [P]),
     "eph_system_set_variant": (I, [P, I, I]),
+    "eph_system_set_graph": (I, [P, I]),
     "eph_particles_create": (I, [P, I64, P, P, D, D, PP]),
     "eph_particles_integrate": (I, [P, D, I64]),
     "eph_particles_download": (I, [P, P, P, P]),

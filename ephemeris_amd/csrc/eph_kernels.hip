@@ -1182,11 +1182,20 @@ __device__ void stage_body(const View &d, const Acc &A, int g, int k, double dt,
     A.st(F_EXJ2, g, A.ld(F_DJ2, g) * t);
 }
 
-__global__ __launch_bounds__(256) void k_stage(View d, int k, double dt, double dt_k, double tadd) {
+// istep != nullptr: the step index comes from device memory (replayed graphs)
+// and the stage time offset dt * (dt_k + i_step) is formed here, with the
+// same two operations as on the host.
+__global__ __launch_bounds__(64) void k_stage(View d, int k, double dt, double dt_k, double tadd,
+                                              const int64_t *__restrict__ istep) {
     const Acc A{d.F, d.nbp};
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= d.nb) return;
+    if (istep) tadd = dt * (dt_k + (double)*istep);
     stage_body(d, A, g, k, dt, dt_k, tadd);
+}
+
+__global__ void k_step_count(int64_t *istep) {
+    if (threadIdx.x == 0) *istep += 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -1536,8 +1545,11 @@ void launch_np(const View &d, int nrows, int max_n, int maxT, const int2 *recv_b
     hipLaunchKernelGGL(k_np_item, dim3((nthr + 63) / 64), dim3(64), 0, st, d, nrows, maxT);
     if (nrecv) hipLaunchKernelGGL(k_np_recv, dim3(nrecv), dim3(kBS), 0, st, d, recv_blk, maxT);
 }
-void launch_stage(const View &d, int k, double dt, double dt_k, double tadd, hipStream_t st) {
-    hipLaunchKernelGGL(k_stage, dim3((d.nb + 255) / 256), dim3(256), 0, st, d, k, dt, dt_k, tadd);
+void launch_stage(const View &d, int k, double dt, double dt_k, double tadd, const int64_t *istep, hipStream_t st) {
+    hipLaunchKernelGGL(k_stage, dim3((d.nb + 63) / 64), dim3(64), 0, st, d, k, dt, dt_k, tadd, istep);
+}
+void launch_step_count(int64_t *istep, hipStream_t st) {
+    hipLaunchKernelGGL(k_step_count, dim3(1), dim3(64), 0, st, istep);
 }
 void launch_small(const View &d, const int *sys_ids, int nsys, int n_steps, double dt, const DtK &dtk,
                   int64_t i_step0, JMain *rec, hipStream_t st) {

@@ -68,7 +68,9 @@ constexpr int kNpFusedMaxN_h = 512, kNpFusedMaxItems_h = 64;
 // parallel, summed in order); max_n = largest system of the batch
 void launch_deform_wpb(const View &d, const int2 *work, int nwork, int max_n, int prepare, hipStream_t st);
 void launch_np(const View &d, int nrows, int max_n, int maxT, const int2 *recv_blk, int nrecv, hipStream_t st);
-void launch_stage(const View &d, int k, double dt, double dt_k, double tadd, hipStream_t st);
+// istep: optional device step counter (graph replays); launch_step_count adds 1
+void launch_stage(const View &d, int k, double dt, double dt_k, double tadd, const int64_t *istep, hipStream_t st);
+void launch_step_count(int64_t *istep, hipStream_t st);
 void launch_extrema(const View &d, int invert, hipStream_t st);
 // per-workgroup schedule of a particle launch: first record entry (25 per
 // step), number of steps (< 0: first acceleration from entry rec_entry0,

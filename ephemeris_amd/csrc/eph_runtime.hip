@@ -119,6 +119,7 @@ struct Scope {
 
 // ---- system ------------------------------------------------------------------
 Batch::~Batch() {
+    drop_graph();
     profiler.collect();
     for (void *p : dev_allocs) (void)hipFree(p);
     for (void *p : class_allocs) (void)hipFree(p);
@@ -332,6 +333,46 @@ int Batch::integrate_small(double dt, int64_t n_step, JMain *rec, int64_t rec_st
     return EPH_OK;
 }
 
+void Batch::drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    gexec = nullptr;
+}
+
+// One RK12 step (25 stage updates and force evaluations, then the step
+// counter) captured as a graph, rebuilt when dt, the summation order or the
+// kernel choice changes.
+int Batch::step_graph(double dt, int mode) {
+    if (gexec && g_dt == dt && g_mode == mode && g_variant == variant) return EPH_OK;
+    drop_graph();
+    if (!d_istep) {
+        HIP_CHECK(hipMalloc(&d_istep, sizeof(int64_t)));
+        dev_allocs.push_back(d_istep);
+    }
+    const View v = view(mode);
+    const DtK dtk = stage_dtk();
+    const bool prof = profiler.enabled;
+    profiler.enabled = false;                          // no event records inside the capture
+    HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    int rc = EPH_OK;
+    for (int k = 1; k <= 25 && !rc; ++k) {
+        launch_stage(v, k, dt, dtk.v[k], 0.0, d_istep, stream);
+        rc = accel(mode, false);
+    }
+    launch_step_count(d_istep, stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(stream, &g);
+    profiler.enabled = prof;
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return fail(EPH_EDEVICE, std::string("step graph capture failed: ") + hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) { gexec = nullptr; return fail(EPH_EDEVICE, "step graph instantiation failed"); }
+    g_dt = dt;
+    g_mode = mode;
+    g_variant = variant;
+    return EPH_OK;
+}
+
 int Batch::integrate(double dt, int64_t n_step, int mode) {
     if (n_step == 0) return EPH_OK;
     if ((mode == 1 || plan[0].n_pool == 0) && all_small() && !force_generic)
@@ -342,6 +383,18 @@ int Batch::integrate(double dt, int64_t n_step, int mode) {
     HIP_CHECK(hipMemcpyAsync(d_t0, t0.data(), sizeof(double) * n_sys, hipMemcpyHostToDevice, stream));
     const View v = view(mode);
     launch_extrema(v, 0, stream);
+    if (use_graph && !profiler.enabled && n_step >= 2) {
+        int rc = step_graph(dt, mode);
+        if (rc) return rc;
+        const int64_t zero = 0;
+        HIP_CHECK(hipMemcpyAsync(d_istep, &zero, sizeof zero, hipMemcpyHostToDevice, stream));
+        for (int64_t i_step = 0; i_step < n_step; ++i_step) HIP_CHECK(hipGraphLaunch(gexec, stream));
+        launch_extrema(v, 1, stream);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(stream));
+        for (int64_t s = 0; s < n_sys; ++s) t_eph[s] = host_add(t_eph[s], mkdd(dt * (double)n_step));
+        return EPH_OK;
+    }
     for (int64_t i_step = 0; i_step < n_step; ++i_step) {
         for (int k = 1; k <= 25; ++k) {
             double dt_k = 0;                                   // RungeKutta.impl:148-152
@@ -352,7 +405,7 @@ int Batch::integrate(double dt, int64_t n_step, int mode) {
             const double tadd = dt * (dt_k + (double)i_step);
             {
                 Scope sc(profiler, 0, stream);
-                launch_stage(v, k, dt, dt_k, tadd, stream);
+                launch_stage(v, k, dt, dt_k, tadd, nullptr, stream);
             }
             int rc = accel(mode, false);
             if (rc) return rc;
@@ -413,6 +466,7 @@ int Batch::build_class_tables(const eph_mass *masses) {
 }
 
 int Batch::upload(const eph_mass *masses) {
+    drop_graph();                        // the class tables are rebuilt below
     host_k2.resize(nb);
     host_td.resize(nb);
     for (int64_t g = 0; g < nb; ++g) { host_k2[g] = masses[g].k2; host_td[g] = masses[g].tide_delay; }
@@ -849,6 +903,12 @@ int eph_system_set_variant(eph_system *sys, int variant, int allow_persistent) {
     if (variant < 0 || variant > 3) return fail(EPH_EINVAL, "variant must be 0 (automatic), 1, 2 or 3");
     B(sys)->variant = variant;
     B(sys)->force_generic = !allow_persistent;
+    return EPH_OK;
+}
+
+int eph_system_set_graph(eph_system *sys, int enable) {
+    if (!sys) return fail(EPH_EINVAL, "null system");
+    B(sys)->use_graph = enable != 0;
     return EPH_OK;
 }
 

@@ -105,6 +105,14 @@ struct Batch {
     int *d_small_ids = nullptr;
     int upload(const eph_mass *masses);
     int download(eph_mass *masses);
+    // step graph (see step_graph)
+    bool use_graph = true;
+    hipGraphExec_t gexec = nullptr;
+    double g_dt = 0;
+    int g_mode = -1, g_variant = -1;
+    int64_t *d_istep = nullptr;
+    int step_graph(double dt, int mode);
+    void drop_graph();
 };
 
 // Massless test particles advanced in the field of one small resident system.

@@ -79,6 +79,10 @@ class GpuSystems:
         pass in its own kernels."""
         _lib.check(_lib.lib().eph_system_set_variant(self.h, int(variant), int(allow_persistent)))
 
+    def set_graph(self, enable=True):
+        """Replay each RK12 step as one captured HIP graph (default on)."""
+        _lib.check(_lib.lib().eph_system_set_graph(self.h, int(bool(enable))))
+
     def profile(self, enable=True):
         _lib.check(_lib.lib().eph_profile_enable(self.h, int(enable)))
 

@@ -135,6 +135,11 @@ int eph_integrate_ex(eph_system *sys, double dt, int64_t n_step, int summation);
  * No counterpart in the reference (its only knob is CUDA_CORES,
  * src/integrators/RungeKutta.cu:13-18, which changes results). */
 int eph_system_set_variant(eph_system *sys, int variant, int allow_persistent);
+/* enable = 1 (default): eph_integrate runs each RK12 step as one replay of a
+ * captured HIP graph (25 stage updates and force evaluations) whenever the
+ * profiler is off and n_step >= 2; 0 launches every kernel from the host.
+ * Results are bit-identical either way. */
+int eph_system_set_graph(eph_system *sys, int enable);
 
 /* ---- massless test particles ------------------------------------------------
  * n particles, each integrated as one more body (appended after the field

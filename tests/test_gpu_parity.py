@@ -144,3 +144,24 @@ def test_integrate_variants_agree(variant, persistent):
         b, _ = ref.download()
         bad = [f for f in a.dtype.names if not bits_equal(a[f], b[f])]
         assert not bad, (name, bad)
+
+
+def test_step_graph_replays_match_eager_launches():
+    """Each RK12 step replayed as a captured graph gives the bits of the
+    host-launched sequence, replay after replay, with eager work between."""
+    print("\n[case] step graph vs eager, full504 and small25 (generic path), 3 rounds of 3 steps", flush=True)
+    for name, tag in [("full504", "pool"), ("small25", "serial")]:
+        d = load_npz(GOLDEN / f"accel_{name}_{tag}.npz")
+        eager, graph, other = _gpu(d), _gpu(d), _gpu(d)
+        for g in (eager, graph, other):
+            g.set_variant(0, allow_persistent=False)
+        eager.set_graph(False)
+        other.set_graph(False)
+        for rnd in range(3):
+            graph.integrate(300.0, 3)
+            other.integrate(300.0, 1)              # eager work between replays
+            eager.integrate(300.0, 3)
+            a, ta = graph.download()
+            b, tb = eager.download()
+            bad = [f for f in a.dtype.names if not bits_equal(a[f], b[f])]
+            assert not bad and bits_equal(ta, tb), (name, rnd, bad)
