@@ -209,6 +209,7 @@ def main():
     n = len(masses)
     g = GpuSystems(masses, None, gps, rings)
     g.integrate(DT, a.warmup)
+    # timed region: host-launched kernels with HIP events around every kernel class
     g.profile(True)
     g.profile_reset()
     t0 = time.perf_counter()
@@ -218,6 +219,11 @@ def main():
     g.profile(False)
     pairs = a.steps * 25 * n * (n - 1)
     value = pairs / el
+    # the same steps again as graph replays (one captured graph per RK12 step)
+    g.integrate(DT, 2)
+    t0 = time.perf_counter()
+    g.integrate(DT, a.steps)
+    el_graph = time.perf_counter() - t0
 
     tot = {k: v[0] * v[1] for k, v in prof.items()}
     dominant = max(tot, key=tot.get)
@@ -236,7 +242,9 @@ def main():
            "vs_baseline": None, "dtype": "f64 (double-double state)", "data": "synthetic",
            "config": {"workload": "full504: 504-body full-model system, RK12 dt=300 s, plain single-system path",
                       "bodies": n, "parallelism": "single GPU"},
-           "rk12_steps_per_s": a.steps / el, "roofline": roofline}
+           "rk12_steps_per_s": a.steps / el, "roofline": roofline,
+           "graph_replay": {"rk12_steps_per_s": a.steps / el_graph, "ms_per_step": el_graph / a.steps * 1e3,
+                            "pair_interactions_per_s": pairs / el_graph}}
     if not a.no_batch:
         b = batched_leg(a.batch_steps, 1, a.clones)
         b["vs_plain_pair_rate"] = b["pair_interactions_per_s"] / value
