@@ -544,6 +544,66 @@ __device__ __forceinline__ void lt_rad(const IMain &I, const JMain &T, const Pai
     da += (T.lum * I.rR2G4c * p.rr3 / I.GM) * p.r;
 }
 
+// Filler for tile slots past the last partner: a unit-mass point at least
+// one unit away from i, so the discarded terms stay finite.
+__device__ __forceinline__ JMain tile_pad(const IMain &I) {
+    JMain T;
+    double *t = (double *)&T;
+#pragma unroll
+    for (int q = 0; q < (int)(sizeof(JMain) / sizeof(double)); ++q) t[q] = 0.0;
+    T.rxh = I.r.x.hi >= 0 ? -1.0 : 1.0;       // |r_j - r_i| >= 1
+    T.ryh = I.r.y.hi;
+    T.rzh = I.r.z.hi;
+    T.GM = 1.0;
+    return T;
+}
+
+// Every term one pair (i, j) adds to i in the thread-pool order
+// (RELATIVITY, ROTATIONAL_TIDAL_DEFORMATION_NANTI_FORCE, LENSE_THIRRING,
+// RADIATION_PRESSURE; mass.impl:82-125, mass.cpp:147-155).
+struct PairTerms { v3 G, dg, rxdg, gj, lt1, lt2, lt3, rad; double rr, tp_dg; };
+__device__ __forceinline__ PairTerms pair_terms(const IMain &I, const JMain &T) {
+    PairTerms o;
+    v3 unused{0, 0, 0};
+    const PairGeo p = relativity(I, T, unused);
+    o.G = p.G;
+    o.rr = p.rr;
+    o.tp_dg = p.tp_dg;
+    o.dg = (T.GM * p.rr3 * p.rr2 * I.R2) * quad_field(I.C, p);
+    o.rxdg = cross(p.r, o.dg);
+    const m3 Cj{v3{T.C[0], T.C[1], T.C[2]}, v3{T.C[3], T.C[4], T.C[5]}, v3{T.C[6], T.C[7], T.C[8]}};
+    o.gj = (T.GM * p.rr3 * p.rr2 * T.R2) * quad_field(Cj, p);
+    const double rcr32 = 2 * T.GM / kC * p.rr3;
+    const v3 GLb = cross(I.GL, p.b);
+    o.lt1 = rcr32 * (GLb - (3 * dot(GLb, p.r) * p.rr2) * p.r);
+    o.lt2 = rcr32 * cross(I.GL, cross(p.r, p.b));
+    const v3 GLj{T.GLx, T.GLy, T.GLz};
+    o.lt3 = cross(rcr32 * (GLj - (3 * dot(GLj, p.r) * p.rr2) * p.r), p.b);
+    o.rad = (T.lum * I.rR2G4c * p.rr3 / I.GM) * p.r;
+    return o;
+}
+
+// Add one pair's terms in the reference order; a pair that is not used adds
+// +0 everywhere, an exact identity (the sums start at +0 and so never hold
+// -0; the running maxima are >= +0).
+struct MainAcc {
+    v3 ga{0, 0, 0}, da{0, 0, 0}, dq{0, 0, 0};
+    double mind, maxi;
+    __device__ __forceinline__ void add(const PairTerms &t, bool use) {
+        const v3 z{0, 0, 0};
+        mind = checked_max_into(mind, use ? t.rr : 0.0);
+        maxi = checked_max_into(maxi, use ? t.tp_dg : 0.0);
+        ga += use ? t.G : z;
+        da += use ? t.dg : z;
+        dq += use ? t.rxdg : z;
+        da += use ? t.gj : z;
+        da -= use ? t.lt1 : z;
+        dq -= use ? t.lt2 : z;
+        da += use ? t.lt3 : z;
+        da += use ? t.rad : z;
+    }
+};
+
 template <int BS>
 __global__ __launch_bounds__(BS) void k_main_pool(View d, const int2 *__restrict__ blk) {
     const Acc A{d.F, d.nbp};
@@ -555,35 +615,30 @@ __global__ __launch_bounds__(BS) void k_main_pool(View d, const int2 *__restrict
     const int gi = off + (valid ? il : 0);
     __shared__ JMain tile[BS];
     const IMain I = load_imain(A, gi);
-    v3 ga{0, 0, 0}, da{0, 0, 0}, dq{0, 0, 0};
-    double mind = A.ld(F_MIND, gi), maxi = A.ld(F_MAXI, gi);
+    MainAcc acc;
+    acc.mind = A.ld(F_MIND, gi);
+    acc.maxi = A.ld(F_MAXI, gi);
     for (int t0 = 0; t0 < n; t0 += BS) {
         __syncthreads();
         if (t0 + (int)threadIdx.x < n) load_jmain(A, tile[threadIdx.x], off + t0 + (int)threadIdx.x);
+        else tile[threadIdx.x] = tile_pad(I);
         __syncthreads();
         if (!valid) continue;
         const int tn = min(BS, n - t0);
-        for (int q = 0; q < tn; ++q) {
-            if (t0 + q == il) continue;
-            const JMain &T = tile[q];
-            const PairGeo p = relativity(I, T, ga);
-            mind = checked_max_into(mind, p.rr);
-            maxi = checked_max_into(maxi, p.tp_dg);
-            // ROTATIONAL_TIDAL_DEFORMATION_NANTI_FORCE, mass.impl:118-125
-            const v3 dg = (T.GM * p.rr3 * p.rr2 * I.R2) * quad_field(I.C, p);
-            da += dg;
-            dq += cross(p.r, dg);
-            const m3 Cj{v3{T.C[0], T.C[1], T.C[2]}, v3{T.C[3], T.C[4], T.C[5]}, v3{T.C[6], T.C[7], T.C[8]}};
-            da += (T.GM * p.rr3 * p.rr2 * T.R2) * quad_field(Cj, p);
-            lt_rad(I, T, p, da, dq);
+        // two independent pairs per iteration for instruction-level parallelism
+        for (int q = 0; q < tn; q += 2) {
+            const PairTerms a = pair_terms(I, tile[q]);
+            const PairTerms b = pair_terms(I, tile[q + 1]);
+            acc.add(a, t0 + q != il);
+            acc.add(b, q + 1 < tn && t0 + q + 1 != il);
         }
     }
     if (!valid) return;
-    A.stv(F_GACC, gi, ga);
-    A.stv(F_DACC, gi, da);
-    A.stv(F_DTQ, gi, dq);
-    A.st(F_MIND, gi, mind);
-    A.st(F_MAXI, gi, maxi);
+    A.stv(F_GACC, gi, acc.ga);
+    A.stv(F_DACC, gi, acc.da);
+    A.stv(F_DTQ, gi, acc.dq);
+    A.st(F_MIND, gi, acc.mind);
+    A.st(F_MAXI, gi, acc.maxi);
     if (sm.nitem == 0) {                                   // FINALIZE_RELATIVITY
         A.st(F_PHI, gi, I.phi * kC2);
         A.stv(F_NACC, gi, A.ldv(F_NACCC, gi) * kC2);
@@ -746,26 +801,15 @@ __global__ __launch_bounds__(64 * W) void k_main_wpb(View d, const int2 *__restr
         if (jl < n && jl != il) {
             JMain T;
             load_jmain(A, T, off + jl);
-            v3 ga{0, 0, 0};
-            const PairGeo p = relativity(I, T, ga);
-            const v3 dg = (T.GM * p.rr3 * p.rr2 * I.R2) * quad_field(I.C, p);
-            const v3 rxdg = cross(p.r, dg);
-            const m3 Cj{v3{T.C[0], T.C[1], T.C[2]}, v3{T.C[3], T.C[4], T.C[5]}, v3{T.C[6], T.C[7], T.C[8]}};
-            const v3 gj = (T.GM * p.rr3 * p.rr2 * T.R2) * quad_field(Cj, p);
-            const double rcr32 = 2 * T.GM / kC * p.rr3;
-            const v3 GLb = cross(I.GL, p.b);
-            const v3 lt1 = rcr32 * (GLb - (3 * dot(GLb, p.r) * p.rr2) * p.r);
-            const v3 lt2 = rcr32 * cross(I.GL, cross(p.r, p.b));
-            const v3 GLj{T.GLx, T.GLy, T.GLz};
-            const v3 lt3 = cross(rcr32 * (GLj - (3 * dot(GLj, p.r) * p.rr2) * p.r), p.b);
-            const v3 rad = (T.lum * I.rR2G4c * p.rr3 / I.GM) * p.r;
-            const double v[kWpbZero] = {p.G.x, p.G.y, p.G.z, dg.x, dg.y, dg.z, gj.x, gj.y, gj.z,
-                                        -lt1.x, -lt1.y, -lt1.z, lt3.x, lt3.y, lt3.z, rad.x, rad.y, rad.z,
-                                        rxdg.x, rxdg.y, rxdg.z, -lt2.x, -lt2.y, -lt2.z};
+            const PairTerms pt = pair_terms(I, T);
+            const double v[kWpbZero] = {pt.G.x, pt.G.y, pt.G.z, pt.dg.x, pt.dg.y, pt.dg.z, pt.gj.x, pt.gj.y, pt.gj.z,
+                                        -pt.lt1.x, -pt.lt1.y, -pt.lt1.z, pt.lt3.x, pt.lt3.y, pt.lt3.z,
+                                        pt.rad.x, pt.rad.y, pt.rad.z, pt.rxdg.x, pt.rxdg.y, pt.rxdg.z,
+                                        -pt.lt2.x, -pt.lt2.y, -pt.lt2.z};
 #pragma unroll
             for (int q = 0; q < kWpbZero; ++q) t[q] = v[q];
-            mind = checked_max_into(mind, p.rr);
-            maxi = checked_max_into(maxi, p.tp_dg);
+            mind = checked_max_into(mind, pt.rr);
+            maxi = checked_max_into(maxi, pt.tp_dg);
         } else {
 #pragma unroll
             for (int q = 0; q < kWpbZero; ++q) t[q] = 0.0;
