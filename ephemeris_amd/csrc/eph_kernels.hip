@@ -544,20 +544,6 @@ __device__ __forceinline__ void lt_rad(const IMain &I, const JMain &T, const Pai
     da += (T.lum * I.rR2G4c * p.rr3 / I.GM) * p.r;
 }
 
-// Filler for tile slots past the last partner: a unit-mass point at least
-// one unit away from i, so the discarded terms stay finite.
-__device__ __forceinline__ JMain tile_pad(const IMain &I) {
-    JMain T;
-    double *t = (double *)&T;
-#pragma unroll
-    for (int q = 0; q < (int)(sizeof(JMain) / sizeof(double)); ++q) t[q] = 0.0;
-    T.rxh = I.r.x.hi >= 0 ? -1.0 : 1.0;       // |r_j - r_i| >= 1
-    T.ryh = I.r.y.hi;
-    T.rzh = I.r.z.hi;
-    T.GM = 1.0;
-    return T;
-}
-
 // Every term one pair (i, j) adds to i in the thread-pool order
 // (RELATIVITY, ROTATIONAL_TIDAL_DEFORMATION_NANTI_FORCE, LENSE_THIRRING,
 // RADIATION_PRESSURE; mass.impl:82-125, mass.cpp:147-155).
@@ -583,24 +569,21 @@ __device__ __forceinline__ PairTerms pair_terms(const IMain &I, const JMain &T) 
     return o;
 }
 
-// Add one pair's terms in the reference order; a pair that is not used adds
-// +0 everywhere, an exact identity (the sums start at +0 and so never hold
-// -0; the running maxima are >= +0).
+// Add one pair's terms in the reference order.
 struct MainAcc {
     v3 ga{0, 0, 0}, da{0, 0, 0}, dq{0, 0, 0};
     double mind, maxi;
-    __device__ __forceinline__ void add(const PairTerms &t, bool use) {
-        const v3 z{0, 0, 0};
-        mind = checked_max_into(mind, use ? t.rr : 0.0);
-        maxi = checked_max_into(maxi, use ? t.tp_dg : 0.0);
-        ga += use ? t.G : z;
-        da += use ? t.dg : z;
-        dq += use ? t.rxdg : z;
-        da += use ? t.gj : z;
-        da -= use ? t.lt1 : z;
-        dq -= use ? t.lt2 : z;
-        da += use ? t.lt3 : z;
-        da += use ? t.rad : z;
+    __device__ __forceinline__ void add(const PairTerms &t) {
+        mind = checked_max_into(mind, t.rr);
+        maxi = checked_max_into(maxi, t.tp_dg);
+        ga += t.G;
+        da += t.dg;
+        dq += t.rxdg;
+        da += t.gj;
+        da -= t.lt1;
+        dq -= t.lt2;
+        da += t.lt3;
+        da += t.rad;
     }
 };
 
@@ -621,16 +604,12 @@ __global__ __launch_bounds__(BS) void k_main_pool(View d, const int2 *__restrict
     for (int t0 = 0; t0 < n; t0 += BS) {
         __syncthreads();
         if (t0 + (int)threadIdx.x < n) load_jmain(A, tile[threadIdx.x], off + t0 + (int)threadIdx.x);
-        else tile[threadIdx.x] = tile_pad(I);
         __syncthreads();
         if (!valid) continue;
         const int tn = min(BS, n - t0);
-        // two independent pairs per iteration for instruction-level parallelism
-        for (int q = 0; q < tn; q += 2) {
-            const PairTerms a = pair_terms(I, tile[q]);
-            const PairTerms b = pair_terms(I, tile[q + 1]);
-            acc.add(a, t0 + q != il);
-            acc.add(b, q + 1 < tn && t0 + q + 1 != il);
+        for (int q = 0; q < tn; ++q) {
+            if (t0 + q == il) continue;
+            acc.add(pair_terms(I, tile[q]));
         }
     }
     if (!valid) return;
@@ -1568,12 +1547,16 @@ void launch_main_wpb(const View &d, const int2 *work, int nwork, int max_n, bool
     else if (max_n > 64) hipLaunchKernelGGL((k_main_wpb<false, 2>), dim3(nwork), dim3(128), 0, st, d, work);
     else hipLaunchKernelGGL((k_main_wpb<false, 1>), dim3(nwork), dim3(64), 0, st, d, work);
 }
+// Wavefronts per body: as many as the partner chunks need while every body's
+// workgroup still fits on the GPU at once (the ordered sums are serial, so
+// extra wavefronts only pay while workgroups would otherwise be idle
+// CUs; a large batch is faster with one wavefront per body).
 void launch_deform_wpb(const View &d, const int2 *work, int nwork, int max_n, int prepare, hipStream_t st) {
     if (!nwork) return;
     const int chunks = (max_n + 63) / 64;
-    if (chunks >= 8) hipLaunchKernelGGL(k_deform_wpb<8>, dim3(nwork), dim3(512), 0, st, d, work, prepare);
-    else if (chunks >= 4) hipLaunchKernelGGL(k_deform_wpb<4>, dim3(nwork), dim3(256), 0, st, d, work, prepare);
-    else if (chunks >= 2) hipLaunchKernelGGL(k_deform_wpb<2>, dim3(nwork), dim3(128), 0, st, d, work, prepare);
+    if (chunks >= 8 && nwork <= 512) hipLaunchKernelGGL(k_deform_wpb<8>, dim3(nwork), dim3(512), 0, st, d, work, prepare);
+    else if (chunks >= 4 && nwork <= 1280) hipLaunchKernelGGL(k_deform_wpb<4>, dim3(nwork), dim3(256), 0, st, d, work, prepare);
+    else if (chunks >= 2 && nwork <= 2816) hipLaunchKernelGGL(k_deform_wpb<2>, dim3(nwork), dim3(128), 0, st, d, work, prepare);
     else hipLaunchKernelGGL(k_deform_wpb<1>, dim3(nwork), dim3(64), 0, st, d, work, prepare);
 }
 void launch_main_serial(const View &d, const int2 *blk, int nblk, hipStream_t st) {
