@@ -1544,7 +1544,7 @@ void launch_main_pool(const View &d, const int2 *blk, int nblk, hipStream_t st) 
 void launch_main_wpb(const View &d, const int2 *work, int nwork, int max_n, bool fused_np, hipStream_t st) {
     if (!nwork) return;
     if (fused_np) hipLaunchKernelGGL((k_main_wpb<true, 2>), dim3(nwork), dim3(128), 0, st, d, work);
-    else if (max_n > 64) hipLaunchKernelGGL((k_main_wpb<false, 2>), dim3(nwork), dim3(128), 0, st, d, work);
+    else if (max_n > 64 && nwork <= 1536) hipLaunchKernelGGL((k_main_wpb<false, 2>), dim3(nwork), dim3(128), 0, st, d, work);
     else hipLaunchKernelGGL((k_main_wpb<false, 1>), dim3(nwork), dim3(64), 0, st, d, work);
 }
 // Wavefronts per body: as many as the partner chunks need while every body's
