@@ -33,6 +33,22 @@ FLOP_MAIN_PAIR = 294          # RELATIVITY 109, tidal (both quadrupoles) 92, Len
 FLOP_DEFORM_PAIR = {"nk2": 54, "ndelay": 89, "damped": 217}
 
 
+PROFILE = ROOT / "profiles" / "kernels_full504_ensemble256_particles1e6.md"
+
+
+def profiled_traffic(kernel_prefix):
+    """Calibrated HBM bytes per dispatch (read + write) of a kernel, from the
+    committed rocprofv3 PMC summary (None if the kernel is not listed)."""
+    try:
+        for line in PROFILE.read_text().splitlines():
+            cols = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cols) == 5 and cols[0].strip("`").startswith(kernel_prefix):
+                return (float(cols[3]) + float(cols[4])) * 1e6
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def load_workload():
     with np.load(ROOT / "tests" / "golden" / "accel_full504_pool.npz", allow_pickle=False) as z:
         return z["masses_in"], z["gps"], z["rings"]
@@ -230,9 +246,11 @@ def main():
     main_ms = prof["main_force_pool"][0]
     flop_main = n * (n - 1) * FLOP_MAIN_PAIR
     achieved = flop_main / (main_ms * 1e-3) / 1e12
-    roofline = {"kernel": "main_force_pool", "dominant_kernel": dominant, "bound": "fp64_valu",
+    roofline = {"kernel": "k_main_wpb<true, 2> (main force + fused non-point pass)", "kernel_class": "main_force_pool",
+                "dominant_kernel": dominant, "bound": "mfma", "compute": "FP64 vector ALU (same 78.6 TFLOP/s roof; no MFMA used)",
                 "achieved": achieved, "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP64_VECTOR_PEAK_TFLOPS, "traffic": None,
+                "frac": achieved / FP64_VECTOR_PEAK_TFLOPS,
+                "traffic": profiled_traffic("void eph::k_main_wpb<true, 2>"), "traffic_unit": "bytes per launch",
                 "flop_per_launch": flop_main, "avg_launch_ms": main_ms,
                 "kernel_ms": {k: round(v[0], 5) for k, v in prof.items()},
                 "kernel_launches": {k: v[1] for k, v in prof.items()}}
