@@ -626,6 +626,92 @@ __global__ __launch_bounds__(BS) void k_main_pool(View d, const int2 *__restrict
 
 
 // ---------------------------------------------------------------------------
+// throughput path for large batches: four lanes per body, 16 bodies per
+// wavefront.  Lane k of a body evaluates partner q0 + k of every group of
+// four and leaves the pair's terms in LDS; lanes 0-2 then add the group's
+// terms, partner by partner, to the x, y or z component of daccel (own and
+// partner's quadrupole, -LT own spin, +LT partner's spin, radiation), gaccel
+// and dtorque (r x dg, -LT torque) -- the reference order of every chain.
+// The self pair and partners past the end add +0, an exact identity (the
+// sums start at +0 and never hold -0); the running maxima are order-free.
+// Four times as many wavefronts as one thread per body keep every SIMD busy
+// and balanced across systems of different sizes.
+// ---------------------------------------------------------------------------
+constexpr int kK4Pitch = 25;
+
+__global__ __launch_bounds__(64) void k_main_k4(View d, const int2 *__restrict__ blk) {
+    const Acc A{d.F, d.nbp};
+    const int s = blk[blockIdx.x].x, i0 = blk[blockIdx.x].y;   // bodies i0 .. i0 + 15
+    const SysMeta sm = d.sys[s];
+    const int n = sm.n, off = sm.off;
+    const int lane = (int)threadIdx.x, b = lane >> 2, k = lane & 3;
+    const int il = i0 + b;
+    const bool valid = il < n;
+    const int gi = off + (valid ? il : 0);
+    __shared__ double term[64 * kK4Pitch];
+    const IMain I = load_imain(A, gi);
+    const int c = k < 3 ? k : 0;
+    double a_d = 0, a_g = 0, a_q = 0;                      // daccel, gaccel, dtorque component c
+    double mind = 0, maxi = 0;
+    const double *grp = term + (b * 4) * kK4Pitch;
+    for (int q0 = 0; q0 < n; q0 += 4) {
+        const int jl = q0 + k;
+        double t[24];
+        if (valid && jl < n && jl != il) {
+            JMain T;
+            load_jmain(A, T, off + jl);
+            const PairTerms pt = pair_terms(I, T);
+            const double v[24] = {pt.G.x, pt.G.y, pt.G.z, pt.dg.x, pt.dg.y, pt.dg.z, pt.gj.x, pt.gj.y, pt.gj.z,
+                                  -pt.lt1.x, -pt.lt1.y, -pt.lt1.z, pt.lt3.x, pt.lt3.y, pt.lt3.z,
+                                  pt.rad.x, pt.rad.y, pt.rad.z, pt.rxdg.x, pt.rxdg.y, pt.rxdg.z,
+                                  -pt.lt2.x, -pt.lt2.y, -pt.lt2.z};
+#pragma unroll
+            for (int r = 0; r < 24; ++r) t[r] = v[r];
+            mind = checked_max_into(mind, pt.rr);
+            maxi = checked_max_into(maxi, pt.tp_dg);
+        } else {
+#pragma unroll
+            for (int r = 0; r < 24; ++r) t[r] = 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < 24; ++r) term[lane * kK4Pitch + r] = t[r];
+        __syncthreads();
+        if (k < 3) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double *tr = grp + u * kK4Pitch;
+                a_d += tr[3 + c];
+                a_d += tr[6 + c];
+                a_d += tr[9 + c];
+                a_d += tr[12 + c];
+                a_d += tr[15 + c];
+                a_g += tr[c];
+                a_q += tr[18 + c];
+                a_q += tr[21 + c];
+            }
+        }
+        __syncthreads();
+    }
+    mind = checked_max_into(mind, __shfl_xor(mind, 1, 64));
+    mind = checked_max_into(mind, __shfl_xor(mind, 2, 64));
+    maxi = checked_max_into(maxi, __shfl_xor(maxi, 1, 64));
+    maxi = checked_max_into(maxi, __shfl_xor(maxi, 2, 64));
+    if (!valid) return;
+    if (k < 3) {
+        A.st(F_GACC + c, gi, a_g);
+        A.st(F_DACC + c, gi, a_d);
+        A.st(F_DTQ + c, gi, a_q);
+    } else {
+        A.st(F_MIND, gi, checked_max_into(A.ld(F_MIND, gi), mind));
+        A.st(F_MAXI, gi, checked_max_into(A.ld(F_MAXI, gi), maxi));
+        if (sm.nitem == 0) {                               // FINALIZE_RELATIVITY
+            A.st(F_PHI, gi, I.phi * kC2);
+            A.stv(F_NACC, gi, A.ldv(F_NACCC, gi) * kC2);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // latency path: one wavefront per body.  Each lane evaluates one partner of
 // a 64-partner tile and leaves that pair's terms in LDS; then one lane per
 // accumulator adds them in partner order, so every sum is formed exactly as
@@ -1540,6 +1626,9 @@ void launch_deform(const View &d, const int4 *blk_light, int n_light, const int4
 }
 void launch_main_pool(const View &d, const int2 *blk, int nblk, hipStream_t st) {
     if (nblk) hipLaunchKernelGGL(k_main_pool<kBS>, dim3(nblk), dim3(kBS), 0, st, d, blk);
+}
+void launch_main_k4(const View &d, const int2 *blk16, int nblk, hipStream_t st) {
+    if (nblk) hipLaunchKernelGGL(k_main_k4, dim3(nblk), dim3(64), 0, st, d, blk16);
 }
 void launch_main_wpb(const View &d, const int2 *work, int nwork, int max_n, bool fused_np, hipStream_t st) {
     if (!nwork) return;

@@ -56,6 +56,8 @@ void set_rk_table(const double *tab650, hipStream_t st);
 void launch_deform(const View &d, const int4 *blk_light, int n_light, const int4 *blk_damped, int n_damped,
                    int prepare, hipStream_t st);
 void launch_main_pool(const View &d, const int2 *blk, int nblk, hipStream_t st);
+// four lanes per body; blk16 = (system, first body) of every 16 bodies
+void launch_main_k4(const View &d, const int2 *blk16, int nblk, hipStream_t st);
 void launch_main_serial(const View &d, const int2 *blk, int nblk, hipStream_t st);
 // latency path, one wavefront per body; work = {system, body index in system}
 // fused_np: the receiver and own-item sums of the non-point pass run inside

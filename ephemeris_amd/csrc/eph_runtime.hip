@@ -194,7 +194,19 @@ int Batch::build_plan(int mode) {
         if (nnp) append_blocks(blk_recv, (int)s, (int)n);
         for (int64_t i = 0; i < n; ++i) work_pool.push_back(make_int2((int)s, (int)i));
     }
+    // 16-body blocks of the four-lanes-per-body kernel, largest systems first
+    std::vector<int2> blk16;
+    {
+        std::vector<int> order;
+        for (int64_t s = 0; s < n_sys; ++s)
+            if (!meta[s].serial) order.push_back((int)s);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sys_n[a] > sys_n[b]; });
+        for (int s : order)
+            for (int i0 = 0; i0 < (int)sys_n[s]; i0 += 16) blk16.push_back(make_int2(s, i0));
+    }
     int rc;
+    if ((rc = dev_upload(*this, pl.d_blk16, blk16))) return rc;
+    pl.n_blk16 = (int)blk16.size();
     if ((rc = dev_upload(*this, pl.d_sys, meta))) return rc;
     if ((rc = dev_upload(*this, pl.d_items, items))) return rc;
     if ((rc = dev_upload(*this, pl.d_blk_pool, blk_pool))) return rc;
@@ -241,7 +253,7 @@ View Batch::view(int mode) const {
 static constexpr int kWpbAutoBodies = 16384;
 
 bool Batch::use_wpb(int mode) const {
-    if (variant == 1) return false;
+    if (variant == 1 || variant == 4) return false;
     if (variant >= 2) return true;
     return plan[mode].n_work_pool > 0 && plan[mode].n_work_pool <= kWpbAutoBodies;
 }
@@ -272,7 +284,8 @@ int Batch::accel(int mode, bool prepare_only) {
     if (pl.n_pool) {
         Scope sc(profiler, 2, stream);
         if (wpb) launch_main_wpb(v, pl.d_work_pool, pl.n_work_pool, pl.max_n, fused_np, stream);
-        else launch_main_pool(v, pl.d_blk_pool, pl.n_pool, stream);
+        else if (variant == 1) launch_main_pool(v, pl.d_blk_pool, pl.n_pool, stream);
+        else launch_main_k4(v, pl.d_blk16, pl.n_blk16, stream);
     }
     if (pl.n_serial) {
         Scope sc(profiler, 3, stream);
@@ -900,7 +913,7 @@ int eph_particles_destroy(eph_particles *ps) {
 
 int eph_system_set_variant(eph_system *sys, int variant, int allow_persistent) {
     if (!sys) return fail(EPH_EINVAL, "null system");
-    if (variant < 0 || variant > 3) return fail(EPH_EINVAL, "variant must be 0 (automatic), 1, 2 or 3");
+    if (variant < 0 || variant > 4) return fail(EPH_EINVAL, "variant must be 0 (automatic) or 1-4");
     B(sys)->variant = variant;
     B(sys)->force_generic = !allow_persistent;
     return EPH_OK;

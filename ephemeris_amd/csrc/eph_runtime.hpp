@@ -50,6 +50,8 @@ struct Plan {
     int2 *d_work_pool = nullptr;     // (system, body) of every body of a pool-order system
     int n_work_pool = 0;
     bool np_fusable = true;          // every pool system fits the fused non-point pass
+    int2 *d_blk16 = nullptr;         // (system, first body) of 16-body blocks, largest systems first
+    int n_blk16 = 0;
 };
 
 struct Batch {

@@ -76,7 +76,7 @@ class GpuSystems:
     def set_variant(self, variant=0, allow_persistent=True):
         """0 automatic, 1 one thread per body, 2 one wavefront per body (non-point
         pass fused where it fits), 3 one wavefront per body with the non-point
-        pass in its own kernels."""
+        pass in its own kernels, 4 four lanes per body."""
         _lib.check(_lib.lib().eph_system_set_variant(self.h, int(variant), int(allow_persistent)))
 
     def set_graph(self, enable=True):

@@ -129,7 +129,8 @@ int eph_integrate_ex(eph_system *sys, double dt, int64_t n_step, int summation);
  * body (throughput; large batches), 2 one wavefront per body (latency; few
  * bodies; the non-point pass runs inside the body's wavefront when every
  * thread-pool-order system has <= 1024 bodies and <= 64 non-point items),
- * 3 as 2 with the non-point pass in its own kernels.
+ * 3 as 2 with the non-point pass in its own kernels, 4 four lanes per body
+ * (throughput; the automatic choice for large batches).
  * allow_persistent = 0 keeps systems of <= 64 bodies off the
  * one-workgroup-per-system path.  All choices give bit-identical results.
  * No counterpart in the reference (its only knob is CUDA_CORES,

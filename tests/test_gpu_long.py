@@ -1,0 +1,65 @@
+"""Long-horizon parity of the 504-body system against the oracle.
+
+The oracle trajectory (tests/golden/make_trajectories.py) loads the system,
+then calls integrate(300 s, chunk) once per checkpoint interval; the GPU run
+replays the same call sequence, because the epoch used for the stage times
+is taken at the start of each call.  Bar: max relative position difference
+<= 1e-12 at every checkpoint (1 simulated year is 105,192 steps).
+
+The 10-day checkpoint runs in the default GPU suite; the full year (about
+five minutes on one MI355X) runs with EPH_LONG=1.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from ephemeris_amd.system import GpuSystems
+from tests.util import GOLDEN, load_npz, position, velocity
+
+pytestmark = pytest.mark.gpu
+
+CHECKPOINTS = [288, 2880, 8640, 28800, 105192]
+POS_TOL = 1e-12
+
+
+def _run(upto):
+    d = load_npz(GOLDEN / "accel_full504_pool.npz")
+    g = GpuSystems(d["masses_in"], None, d["gps"], d["rings"])
+    done = 0
+    rows = []
+    for ck in CHECKPOINTS:
+        if ck > upto:
+            break
+        g.integrate(300.0, ck - done)
+        done = ck
+        path = GOLDEN / f"traj_full504_{ck}.npz"
+        if not path.exists():
+            continue
+        ref = load_npz(path)
+        m, t = g.download()
+        r, rr = position(m), position(ref)
+        dr = np.max(np.linalg.norm(r - rr, axis=1) / np.linalg.norm(rr, axis=1))
+        dv = np.max(np.linalg.norm(velocity(m) - velocity(ref), axis=1) /
+                    np.maximum(np.linalg.norm(velocity(ref), axis=1), 1e-30))
+        da = np.max(np.abs(r - rr))
+        print(f"full504 after {ck} steps ({ck / 288:.2f} days): max rel dr={dr:.3e} dv={dv:.3e} "
+              f"max |dr|={da:.3e} m, t_eph equal={t[0].sum() == float(ref['t_eph'])}", flush=True)
+        rows.append((ck, dr))
+        assert t[0].sum() == float(ref["t_eph"])
+    return rows
+
+
+def test_full504_ten_days():
+    print("\n[case] full504 to 10 days (288 + 2592 steps)", flush=True)
+    rows = _run(2880)
+    assert rows and all(dr <= POS_TOL for _, dr in rows), rows
+
+
+@pytest.mark.slow
+@pytest.mark.timeout(1200)
+@pytest.mark.skipif(os.environ.get("EPH_LONG") != "1", reason="one simulated year takes minutes; set EPH_LONG=1")
+def test_full504_one_year():
+    print("\n[case] full504 to one year (105192 steps)", flush=True)
+    rows = _run(105192)
+    assert rows and all(dr <= POS_TOL for _, dr in rows), rows

@@ -115,7 +115,7 @@ def test_integrate_is_deterministic_and_chunk_consistent():
     assert bits_equal(ma, mb)
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4])
 @pytest.mark.parametrize("name,tag", [("full504", "pool"), ("small25", "serial")])
 def test_kernel_variants_bitwise(variant, name, tag):
     """Per-thread and per-wavefront kernels reproduce the reference bits."""
@@ -129,7 +129,7 @@ def test_kernel_variants_bitwise(variant, name, tag):
     assert not bad, bad
 
 
-@pytest.mark.parametrize("variant,persistent", [(1, False), (2, False), (3, False), (0, True)])
+@pytest.mark.parametrize("variant,persistent", [(1, False), (2, False), (3, False), (4, False), (0, True)])
 def test_integrate_variants_agree(variant, persistent):
     print(f"\n[case] integrate full504 and small25 12 steps, variant {variant}", flush=True)
     for name, tag in [("full504", "pool"), ("small25", "serial")]:
