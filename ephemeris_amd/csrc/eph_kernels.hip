@@ -392,6 +392,94 @@ __global__ __launch_bounds__(BS) void k_deform_light(View d, const int4 *__restr
     deform_write(A, gi, prepare, A.lddv(F_V, gi), phi, nacc, C, GI, w);
 }
 
+// The same for large batches with four lanes per body (16 bodies per
+// wavefront): lane k evaluates partner q0 + k of every group of four and
+// leaves its terms in LDS; lane c then adds the group's terms, partner by
+// partner, to chains c, c + 4, c + 8 (and lane 0 to chain 12), where the
+// chains are phi (0), naccel (1-3) and the tidal matrix (4-12), the
+// subtracted terms stored negated.  The self pair and partners past the end
+// add +0, an exact identity.
+constexpr int kDefK4Pitch = 13;
+__global__ __launch_bounds__(64) void k_deform_k4(View d, const int4 *__restrict__ blk, int prepare) {
+    const Acc A{d.F, d.nbp};
+    const int4 bk = blk[blockIdx.x];
+    const SysMeta sm = d.sys[bk.x];
+    const int n = sm.n, off = sm.off;
+    const int lane = (int)threadIdx.x, b = lane >> 2, k = lane & 3;
+    const bool valid = b < bk.z;
+    const int gi = valid ? d.perm[off + bk.y + b] : off;
+    const int il = gi - off;
+    __shared__ double term[64 * kDefK4Pitch];
+    const ddv3 ri = A.lddv(F_R, gi);
+    const double sR2 = A.ld(F_SR2, gi), k2 = A.ld(F_K2, gi);
+    const bool ndelay = valid && k2 != 0;
+    const bool any_nd = __ballot(ndelay) != 0;             // wave-uniform
+    double acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
+    const double *grp = term + (b * 4) * kDefK4Pitch;
+    for (int q0 = 0; q0 < n; q0 += 4) {
+        const int jl = q0 + k;
+        double t[13];
+#pragma unroll
+        for (int r = 0; r < 13; ++r) t[r] = 0.0;
+        if (valid && jl < n && jl != il) {
+            const int gj = off + jl;
+            const v3 r = diff(A.lddv(F_R, gj), ri);
+            const double rr2 = 1 / smax(dot(r, r), sR2 + A.ld(F_SR2, gj));
+            const double rr = sqrt(rr2);
+            const double tp_dphi = rr * A.ld(F_GM, gj);
+            const double tp_dg = rr2 * tp_dphi;
+            const v3 na = tp_dg * r;
+            t[0] = -tp_dphi;                               // _NK2 (mass.impl:38-46) / _NDELAY (:25-36)
+            t[1] = na.x; t[2] = na.y; t[3] = na.z;
+            if (ndelay) {
+                const double fmj = -3 * tp_dg;
+                const m3 tm = (fmj * k2) * subdiag(outer(rr2 * r, r), 1.0 / 3);
+                const double *tp = &tm.x.x;
+#pragma unroll
+                for (int e = 0; e < 9; ++e) t[4 + e] = -tp[e];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 13; ++r) term[lane * kDefK4Pitch + r] = t[r];
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double *tr = grp + u * kDefK4Pitch;
+            acc0 += tr[k];
+            if (any_nd) {
+                acc1 += tr[4 + k];
+                acc2 += tr[8 + k];
+                if (k == 0) acc3 += tr[12];
+            }
+        }
+        __syncthreads();
+    }
+    // gather the body's chains into its lane 0
+    const int base = lane & ~3;
+    double v[3][3];
+#pragma unroll
+    for (int o = 1; o <= 3; ++o) {
+        v[0][o - 1] = __shfl(acc0, base + o, 64);
+        v[1][o - 1] = __shfl(acc1, base + o, 64);
+        v[2][o - 1] = __shfl(acc2, base + o, 64);
+    }
+    if (!valid || k != 0) return;
+    const double phi = acc0;
+    const v3 nacc{v[0][0], v[0][1], v[0][2]};
+    const m3 Csum{v3{acc1, v[1][0], v[1][1]}, v3{v[1][2], acc2, v[2][0]}, v3{v[2][1], v[2][2], acc3}};
+    Harmonics H;
+    H.load(A, gi);
+    v3 w = A.ldv(F_W, gi);
+    const ddv3 GL = A.lddv(F_GL, gi);
+    m3 C, GI;
+    int iters = 4;                                         // MAX_ANGULAR_VELOCITY_ITER
+    for (;;) {
+        H.apply(Csum, w, C, GI);
+        if (!prepare || update_spin(A, gi, GI, w, GL) || --iters == 0) break;
+    }
+    deform_write(A, gi, prepare, A.lddv(F_V, gi), phi, nacc, C, GI, w);
+}
+
 // Bodies with constant-time-lag tides (k2 != 0, tide_delay != 0): the tidal
 // phase depends on the spin, so each fixed-point pass re-sums the partners
 // (DAMPED_TIDAL_DEFORMATION_MATRIX, mass.impl:1-23).
@@ -1619,6 +1707,9 @@ void set_rk_table(const double *tab650, hipStream_t st) {
     (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(c_rk), tab650, sizeof(double) * 650, 0, hipMemcpyHostToDevice, st);
 }
 
+void launch_deform_k4(const View &d, const int4 *blk16, int nblk, int prepare, hipStream_t st) {
+    if (nblk) hipLaunchKernelGGL(k_deform_k4, dim3(nblk), dim3(64), 0, st, d, blk16, prepare);
+}
 void launch_deform(const View &d, const int4 *blk_light, int n_light, const int4 *blk_damped, int n_damped,
                    int prepare, hipStream_t st) {
     if (n_damped) hipLaunchKernelGGL(k_deform_damped<kBS>, dim3(n_damped), dim3(kBS), 0, st, d, blk_damped, prepare);

@@ -272,7 +272,7 @@ int Batch::accel(int mode, bool prepare_only) {
             // constant-time-lag bodies re-sum their partners once per spin pass:
             // one wavefront each, the rest one thread each
             launch_deform_wpb(v, d_work_damped, n_work_damped, max_sys_n, prepare_only ? 0 : 1, stream);
-            launch_deform(v, d_blk_light, n_blk_light, nullptr, 0, prepare_only ? 0 : 1, stream);
+            launch_deform_k4(v, d_blk_light16, n_blk_light16, prepare_only ? 0 : 1, stream);
         }
     }
     if (prepare_only) return EPH_OK;
@@ -439,6 +439,7 @@ int Batch::build_class_tables(const eph_mass *masses) {
     std::vector<int32_t> perm(nb);
     std::vector<int4> light, damped;
     std::vector<int2> damped_work;       // (system, body) of every constant-time-lag body
+    std::vector<int4> light16;           // 16-body blocks of the other bodies
     for (int64_t s = 0; s < n_sys; ++s) {
         int64_t k = sys_off[s];
         for (int pass = 0; pass < 2; ++pass)
@@ -455,6 +456,8 @@ int Batch::build_class_tables(const eph_mass *masses) {
         for (int64_t q = 0; q < nd; q += kBS) damped.push_back(make_int4((int)s, (int)q, (int)std::min<int64_t>(kBS, nd - q), 0));
         for (int64_t q = nd; q < sys_n[s]; q += kBS)
             light.push_back(make_int4((int)s, (int)q, (int)std::min<int64_t>(kBS, sys_n[s] - q), 0));
+        for (int64_t q = nd; q < sys_n[s]; q += 16)
+            light16.push_back(make_int4((int)s, (int)q, (int)std::min<int64_t>(16, sys_n[s] - q), 0));
     }
     HIP_CHECK(hipStreamSynchronize(stream));
     for (void *p : class_allocs) (void)hipFree(p);
@@ -472,6 +475,8 @@ int Batch::build_class_tables(const eph_mass *masses) {
     if ((rc = up(d_blk_light, light))) return rc;
     if ((rc = up(d_blk_damped, damped))) return rc;
     if ((rc = up(d_work_damped, damped_work))) return rc;
+    if ((rc = up(d_blk_light16, light16))) return rc;
+    n_blk_light16 = (int)light16.size();
     n_blk_light = (int)light.size();
     n_blk_damped = (int)damped.size();
     n_work_damped = (int)damped_work.size();

@@ -81,6 +81,8 @@ struct Batch {
     int n_blk_light = 0, n_blk_damped = 0;
     int2 *d_work_damped = nullptr;
     int n_work_damped = 0;
+    int4 *d_blk_light16 = nullptr;
+    int n_blk_light16 = 0;
     std::vector<void *> class_allocs;
     Plan plan[2];
     std::vector<void *> dev_allocs;
