@@ -83,13 +83,14 @@ def batched_leg(steps, warmup, n_clones):
     el = time.perf_counter() - t0
     prof = g.profile_read()
     g.profile(False)
+    variant = g.kernel_variant()
     pairs = ens.pairs_per_eval()
     main_ms = prof["main_force_pool"][0]
     achieved = pairs * FLOP_MAIN_PAIR / (main_ms * 1e-3) / 1e12
     return {"members": ens.n_members, "bodies": int(ens.sys_n.sum()), "pairs_per_eval": pairs,
             "steps": steps, "pair_interactions_per_s": steps * 25 * pairs / el,
             "rk12_member_steps_per_s": steps * ens.n_members / el, "ms_per_step": el / steps * 1e3,
-            "load_s": t_load,
+            "load_s": t_load, "kernel_variant": {2: "one wavefront per body", 4: "four lanes per body"}.get(variant, variant),
             "roofline_main_force": {"achieved": achieved, "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
                                     "frac": achieved / FP64_VECTOR_PEAK_TFLOPS, "avg_launch_ms": main_ms},
             "kernel_ms": {k: round(v[0], 5) for k, v in prof.items()}}

@@ -36,6 +36,8 @@ SIGNATURES = {
     "eph_profile_reset": (I, [P]),
     "eph_system_set_variant": (I, [P, I, I]),
     "eph_system_set_graph": (I, [P, I]),
+    "eph_system_set_autotune": (I, [P, I]),
+    "eph_system_kernel_variant": (I, [P]),
     "eph_particles_create": (I, [P, I64, P, P, D, D, PP]),
     "eph_particles_integrate": (I, [P, D, I64]),
     "eph_particles_download": (I, [P, P, P, P]),

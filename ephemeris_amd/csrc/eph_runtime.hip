@@ -4,6 +4,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <chrono>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -253,12 +258,14 @@ View Batch::view(int mode) const {
 static constexpr int kWpbAutoBodies = 16384;
 
 bool Batch::use_wpb(int mode) const {
+    const int variant = eff_variant();
     if (variant == 1 || variant == 4) return false;
     if (variant >= 2) return true;
     return plan[mode].n_work_pool > 0 && plan[mode].n_work_pool <= kWpbAutoBodies;
 }
 
 int Batch::accel(int mode, bool prepare_only) {
+    const int variant = eff_variant();
     const View v = view(mode);
     const Plan &pl = plan[mode];
     const bool wpb = use_wpb(mode);
@@ -355,7 +362,7 @@ void Batch::drop_graph() {
 // counter) captured as a graph, rebuilt when dt, the summation order or the
 // kernel choice changes.
 int Batch::step_graph(double dt, int mode) {
-    if (gexec && g_dt == dt && g_mode == mode && g_variant == variant) return EPH_OK;
+    if (gexec && g_dt == dt && g_mode == mode && g_variant == eff_variant()) return EPH_OK;
     drop_graph();
     if (!d_istep) {
         HIP_CHECK(hipMalloc(&d_istep, sizeof(int64_t)));
@@ -382,7 +389,58 @@ int Batch::step_graph(double dt, int mode) {
     if (ei != hipSuccess) { gexec = nullptr; return fail(EPH_EDEVICE, "step graph instantiation failed"); }
     g_dt = dt;
     g_mode = mode;
-    g_variant = variant;
+    g_variant = eff_variant();
+    return EPH_OK;
+}
+
+// ---- autotuner ------------------------------------------------------------------
+// For a batch in automatic mode with enough thread-pool-order bodies, time
+// one wavefront per body against four lanes per body on a device copy of the
+// batch (results are bit-identical, so only speed differs) and keep the
+// faster; choices are cached per shape bucket for the life of the process.
+namespace {
+constexpr int kTuneMinBodies = 2048;
+std::mutex g_tune_mu;
+std::map<std::array<int, 4>, int> g_tune_cache;
+int log2_bucket(int64_t x) { int b = 0; while (x > 1) { x >>= 1; ++b; } return b; }
+}  // namespace
+
+int Batch::autotune(double dt, int mode) {
+    tune_done = true;
+    const Plan &pl = plan[mode];
+    if (pl.n_work_pool < kTuneMinBodies) return EPH_OK;
+    const std::array<int, 4> key{log2_bucket(pl.n_work_pool), (pl.max_n + 63) / 64, log2_bucket(n_sys),
+                                 pl.nrows > 0 ? 1 : 0};
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) { tuned = it->second; return EPH_OK; }
+    }
+    std::vector<int> all((size_t)n_sys);
+    for (int64_t k = 0; k < n_sys; ++k) all[k] = (int)k;
+    int best = 0;
+    double best_s = 1e300;
+    for (int cand : {2, 4}) {
+        Batch *c = nullptr;
+        if (regroup_batch(this, all, 0, nullptr, nullptr, nullptr, 0, nullptr, 0, nullptr, &c) || !c) {
+            g_last_error.clear();                          // not enough memory for a copy: keep the heuristic
+            return EPH_OK;
+        }
+        std::unique_ptr<Batch> own(c);
+        c->variant = cand;
+        c->use_graph = use_graph;
+        c->force_generic = true;
+        c->tune_done = true;
+        int rc = c->integrate(dt, 1, mode);                // warm-up (and graph capture)
+        const auto t0 = std::chrono::steady_clock::now();
+        if (!rc) rc = c->integrate(dt, 2, mode);
+        const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (rc) return rc;
+        if (el < best_s) { best_s = el; best = cand; }
+    }
+    tuned = best;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = best;
     return EPH_OK;
 }
 
@@ -390,6 +448,10 @@ int Batch::integrate(double dt, int64_t n_step, int mode) {
     if (n_step == 0) return EPH_OK;
     if ((mode == 1 || plan[0].n_pool == 0) && all_small() && !force_generic)
         return integrate_small(dt, n_step, nullptr, 0, nullptr, nullptr);
+    if (variant == 0 && use_autotune && !tune_done) {
+        int rc = autotune(n_step < 0 ? -dt : dt, mode);
+        if (rc) return rc;
+    }
     if (n_step < 0) { n_step = -n_step; dt = -dt; }
     std::vector<double> t0(n_sys);
     for (int64_t s = 0; s < n_sys; ++s) t0[s] = todbl(t_eph[s]);
@@ -921,7 +983,19 @@ int eph_system_set_variant(eph_system *sys, int variant, int allow_persistent) {
     if (variant < 0 || variant > 4) return fail(EPH_EINVAL, "variant must be 0 (automatic) or 1-4");
     B(sys)->variant = variant;
     B(sys)->force_generic = !allow_persistent;
+    B(sys)->drop_graph();
     return EPH_OK;
+}
+
+int eph_system_set_autotune(eph_system *sys, int enable) {
+    if (!sys) return fail(EPH_EINVAL, "null system");
+    B(sys)->use_autotune = enable != 0;
+    return EPH_OK;
+}
+
+int eph_system_kernel_variant(const eph_system *sys) {
+    if (!sys) return fail(EPH_EINVAL, "null system");
+    return B(sys)->eff_variant();
 }
 
 int eph_system_set_graph(eph_system *sys, int enable) {

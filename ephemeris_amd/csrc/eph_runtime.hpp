@@ -91,6 +91,11 @@ struct Batch {
     // kernel variant of the j-loop passes: 0 = automatic, 1 = one thread per
     // body, 2 = one wavefront per body
     int variant = 0;
+    // automatic choice made by the autotuner (0: static heuristic)
+    int tuned = 0;
+    bool use_autotune = true, tune_done = false;
+    int eff_variant() const { return variant ? variant : tuned; }
+    int autotune(double dt, int mode);
     int2 *d_work_all = nullptr;
     int n_work_all = 0;
     bool use_wpb(int mode) const;

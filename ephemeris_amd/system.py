@@ -79,6 +79,13 @@ class GpuSystems:
         pass in its own kernels, 4 four lanes per body."""
         _lib.check(_lib.lib().eph_system_set_variant(self.h, int(variant), int(allow_persistent)))
 
+    def set_autotune(self, enable=True):
+        """Time the kernel variants on a device copy at the first integrate (default on)."""
+        _lib.check(_lib.lib().eph_system_set_autotune(self.h, int(bool(enable))))
+
+    def kernel_variant(self):
+        return _lib.check(_lib.lib().eph_system_kernel_variant(self.h))
+
     def set_graph(self, enable=True):
         """Replay each RK12 step as one captured HIP graph (default on)."""
         _lib.check(_lib.lib().eph_system_set_graph(self.h, int(bool(enable))))

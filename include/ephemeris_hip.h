@@ -136,6 +136,13 @@ int eph_integrate_ex(eph_system *sys, double dt, int64_t n_step, int summation);
  * No counterpart in the reference (its only knob is CUDA_CORES,
  * src/integrators/RungeKutta.cu:13-18, which changes results). */
 int eph_system_set_variant(eph_system *sys, int variant, int allow_persistent);
+/* Autotuner (default on): in automatic mode, the first eph_integrate of a
+ * batch with >= 2048 thread-pool-order bodies times variants 2 and 4 for a
+ * few steps on a device copy of the batch and keeps the faster (the choice
+ * is cached per batch shape for the process).  Results do not depend on it.
+ * eph_system_kernel_variant returns the variant in use (0: heuristic). */
+int eph_system_set_autotune(eph_system *sys, int enable);
+int eph_system_kernel_variant(const eph_system *sys);
 /* enable = 1 (default): eph_integrate runs each RK12 step as one replay of a
  * captured HIP graph (25 stage updates and force evaluations) whenever the
  * profiler is off and n_step >= 2; 0 launches every kernel from the host.

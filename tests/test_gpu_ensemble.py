@@ -47,3 +47,24 @@ def test_members_match_oracle(members):
         dr = np.max(np.linalg.norm(position(mine) - position(ref), axis=1) / np.linalg.norm(position(ref), axis=1))
         print(f"member {k} (N={len(ref)}): max rel dr = {dr:.3e}", flush=True)
         assert dr <= 1e-13
+
+
+def test_autotuned_batch_matches_fixed_variant(tmp_path_factory):
+    """The autotuner (on a batch above its size floor) only picks a kernel
+    variant; the trajectory bits equal those of a fixed variant."""
+    print("\n[case] autotuned 16-member batch vs variant 1, 3 steps", flush=True)
+    specs = gen.make_ensemble(16, seed=98, n_lo=150, n_hi=300)
+    cfgs = ensemble.write_members(specs, tmp_path_factory.mktemp("tune"))
+    masses, sys_n, gps, rings, t0 = ensemble.load_members(cfgs)
+    assert sys_n.sum() >= 2048
+    tuned = GpuSystems(masses, sys_n, gps, rings, t0)
+    fixed = GpuSystems(masses, sys_n, gps, rings, t0)
+    fixed.set_variant(1)
+    tuned.integrate(300.0, 3)
+    fixed.integrate(300.0, 3)
+    print("autotuner chose variant", tuned.kernel_variant(), flush=True)
+    assert tuned.kernel_variant() in (2, 4)
+    a, ta = tuned.download()
+    b, tb = fixed.download()
+    bad = [f for f in a.dtype.names if not bits_equal(a[f], b[f])]
+    assert not bad and bits_equal(ta, tb), bad
