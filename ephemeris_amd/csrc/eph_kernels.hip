@@ -439,8 +439,13 @@ __global__ __launch_bounds__(64) void k_deform_k4(View d, const int4 *__restrict
                 for (int e = 0; e < 9; ++e) t[4 + e] = -tp[e];
             }
         }
+        if (any_nd) {
 #pragma unroll
-        for (int r = 0; r < 13; ++r) term[lane * kDefK4Pitch + r] = t[r];
+            for (int r = 0; r < 13; ++r) term[lane * kDefK4Pitch + r] = t[r];
+        } else {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) term[lane * kDefK4Pitch + r] = t[r];
+        }
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -1023,7 +1028,7 @@ constexpr int kWpbDefPitch = 65;
 // W waves per body: in each round wave w evaluates partner chunk
 // (round * W + w); wave 0 then adds the W chunks in partner order.
 template <int W>
-__global__ __launch_bounds__(64 * W) void k_deform_wpb(View d, const int2 *__restrict__ work, int prepare) {
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 1 ? 3 : 1))) void k_deform_wpb(View d, const int2 *__restrict__ work, int prepare) {
     const Acc A{d.F, d.nbp};
     const int s = work[blockIdx.x].x, il = work[blockIdx.x].y;
     const SysMeta sm = d.sys[s];
@@ -1041,20 +1046,39 @@ __global__ __launch_bounds__(64 * W) void k_deform_wpb(View d, const int2 *__res
     Harmonics H;
     m3 C, GI;
     int iters_left = 4;
+    // the spin-independent part of a pair; when every partner has its own
+    // lane (one round), it is formed once and kept across spin passes
+    const bool one_round = W > 1 && n <= 64 * W;           // (one wavefront: keep registers for occupancy)
+    v3 pr{0, 0, 0}, pv{0, 0, 0};
+    double prr2 = 0, ptp_dphi = 0, ptp_dg = 0, ptdf = 0;
+    auto pair_geo = [&](int jl) {
+        const int gj = off + jl;
+        pr = diff(A.lddv(F_R, gj), ri);
+        prr2 = 1 / smax(dot(pr, pr), sR2 + A.ld(F_SR2, gj));
+        const double rr = sqrt(prr2);
+        ptp_dphi = rr * A.ld(F_GM, gj);
+        ptp_dg = prr2 * ptp_dphi;
+        if (damped) {
+            pv = diff(A.lddv(F_V, gj), vi);
+            ptdf = A.ld(F_TDF, gj);
+        }
+    };
+    if (one_round) {
+        const int jl = 64 * wv + lane;
+        if (jl < n && jl != il) pair_geo(jl);
+    }
+    bool first_pass = true;
     for (;;) {
-        double acc = 0;
+        double acc = (wv == 0 && lane < 4 && !first_pass) ? sums[lane] : 0.0;
         for (int t0 = 0; t0 < n; t0 += 64 * W) {
             const int jl = t0 + 64 * wv + lane;
             double t[13];
 #pragma unroll
             for (int q = 0; q < 13; ++q) t[q] = 0.0;
             if (jl < n && jl != il) {
-                const int gj = off + jl;
-                const v3 r = diff(A.lddv(F_R, gj), ri);
-                const double rr2 = 1 / smax(dot(r, r), sR2 + A.ld(F_SR2, gj));
-                const double rr = sqrt(rr2);
-                const double tp_dphi = rr * A.ld(F_GM, gj);
-                const double tp_dg = rr2 * tp_dphi;
+                if (!one_round) pair_geo(jl);
+                const v3 r = pr;
+                const double rr2 = prr2, tp_dphi = ptp_dphi, tp_dg = ptp_dg;
                 const v3 na = tp_dg * r;
                 t[0] = -tp_dphi;
                 t[1] = na.x; t[2] = na.y; t[3] = na.z;
@@ -1065,11 +1089,11 @@ __global__ __launch_bounds__(64 * W) void k_deform_wpb(View d, const int2 *__res
 #pragma unroll
                     for (int q = 0; q < 9; ++q) t[4 + q] = -tp[q];
                 } else if (damped) {
-                    const v3 v = diff(A.lddv(F_V, gj), vi);
+                    const v3 v = pv;
                     const double fmj = -3 * tp_dg;
                     v3 dw = rr2 * cross(r, cross(w, r) - v);
                     double dwt = normalize(dw);
-                    dwt *= td * A.ld(F_TDF, gj);
+                    dwt *= td * ptdf;
                     const double ecwt2 = 1 + 4 * dwt * dwt,
                                  secwt2 = sqrt(ecwt2),
                                  x0 = sqrt((1 + secwt2) / (2 * ecwt2)),
@@ -1088,7 +1112,7 @@ __global__ __launch_bounds__(64 * W) void k_deform_wpb(View d, const int2 *__res
                 for (int q = 0; q < 13; ++q) term[wv][q * kWpbDefPitch + lane] = t[q];
             }
             __syncthreads();
-            if (wv == 0 && lane < nacc_chains) {
+            if (wv == 0 && lane < nacc_chains && (first_pass || lane >= 4)) {
                 const int nw = min(W, (n - t0 + 63) / 64);
                 for (int u = 0; u < nw; ++u) {
                     const double *row = term[u] + lane * kWpbDefPitch;
@@ -1098,7 +1122,9 @@ __global__ __launch_bounds__(64 * W) void k_deform_wpb(View d, const int2 *__res
             }
             __syncthreads();
         }
-        if (wv == 0 && lane < 13) sums[lane] = acc;
+        __syncthreads();
+        if (wv == 0 && lane < 13) sums[lane] = acc;        // phi, naccel: the first pass's sums, unchanged
+        first_pass = false;
         __syncthreads();
         if (threadIdx.x == 0) {
             const m3 Csum{v3{sums[4], sums[5], sums[6]}, v3{sums[7], sums[8], sums[9]}, v3{sums[10], sums[11], sums[12]}};
