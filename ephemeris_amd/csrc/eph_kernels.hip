@@ -262,6 +262,35 @@ __device__ void finalize_body(const View &d, const Acc &A, int g, v3 da, v3 dq) 
     finalize_with(d, A, g, A.ldv(F_GACC, g), da, dq);
 }
 
+// Field of non-point item `row` at partner jl of its system, per unit GM of
+// the item's body (k_np_field; also run by extra workgroups of the deform
+// launches, since it needs only the stage's positions and frames).
+__device__ __forceinline__ void np_field_one(const View &d, const Acc &A, int row, int jl) {
+    const NPItem it = d.items[row];
+    const SysMeta sm = d.sys[it.sys];
+    if (jl >= sm.n) return;
+    const int gj = sm.off + jl;
+    v3 an{0, 0, 0};
+    if (gj != it.body) {
+        const v3 r = diff(A.lddv(F_R, gj), A.lddv(F_R, it.body));
+        an = nonpoint_field(d, A, it.body, it.kind, it.model, r);
+    }
+    double *o = d.an + 3 * ((int64_t)it.an_off + jl);
+    o[0] = an.x; o[1] = an.y; o[2] = an.z;
+}
+
+// Extra workgroups appended to a deform launch: workgroup `x` past the
+// deform ones covers partners [c * blockDim, (c + 1) * blockDim) of item row
+// x / np.chunks.
+struct NpTail { int ndef, nrows, chunks; };
+__device__ __forceinline__ bool np_tail(const View &d, const NpTail &np) {
+    if ((int)blockIdx.x < np.ndef) return false;
+    const int x = (int)blockIdx.x - np.ndef;
+    const int row = x / np.chunks, c = x % np.chunks;
+    if (row < np.nrows) np_field_one(d, Acc{d.F, d.nbp}, row, c * (int)blockDim.x + (int)threadIdx.x);
+    return true;
+}
+
 // ---------------------------------------------------------------------------
 // deform + spin fixed point + PREPARE_RELATIVITY (mass.cpp:78-135)
 // ---------------------------------------------------------------------------
@@ -400,7 +429,8 @@ __global__ __launch_bounds__(BS) void k_deform_light(View d, const int4 *__restr
 // subtracted terms stored negated.  The self pair and partners past the end
 // add +0, an exact identity.
 constexpr int kDefK4Pitch = 13;
-__global__ __launch_bounds__(64) void k_deform_k4(View d, const int4 *__restrict__ blk, int prepare) {
+__global__ __launch_bounds__(64) void k_deform_k4(View d, const int4 *__restrict__ blk, int prepare, NpTail np) {
+    if (np_tail(d, np)) return;
     const Acc A{d.F, d.nbp};
     const int4 bk = blk[blockIdx.x];
     const SysMeta sm = d.sys[bk.x];
@@ -1028,7 +1058,8 @@ constexpr int kWpbDefPitch = 65;
 // W waves per body: in each round wave w evaluates partner chunk
 // (round * W + w); wave 0 then adds the W chunks in partner order.
 template <int W>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 1 ? 3 : 1))) void k_deform_wpb(View d, const int2 *__restrict__ work, int prepare) {
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 1 ? 3 : 1))) void k_deform_wpb(View d, const int2 *__restrict__ work, int prepare, NpTail np) {
+    if (np_tail(d, np)) return;
     const Acc A{d.F, d.nbp};
     const int s = work[blockIdx.x].x, il = work[blockIdx.x].y;
     const SysMeta sm = d.sys[s];
@@ -1245,18 +1276,7 @@ __global__ void k_np_field(View d, int nrows) {
     const Acc A{d.F, d.nbp};
     const int row = blockIdx.y;
     if (row >= nrows) return;
-    const NPItem it = d.items[row];
-    const SysMeta sm = d.sys[it.sys];
-    const int jl = blockIdx.x * blockDim.x + threadIdx.x;
-    if (jl >= sm.n) return;
-    const int gj = sm.off + jl;
-    v3 an{0, 0, 0};
-    if (gj != it.body) {
-        const v3 r = diff(A.lddv(F_R, gj), A.lddv(F_R, it.body));
-        an = nonpoint_field(d, A, it.body, it.kind, it.model, r);
-    }
-    double *o = d.an + 3 * ((int64_t)it.an_off + jl);
-    o[0] = an.x; o[1] = an.y; o[2] = an.z;
+    np_field_one(d, A, row, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 __global__ void k_np_item(View d, int nrows, int maxT) {
@@ -1733,8 +1753,10 @@ void set_rk_table(const double *tab650, hipStream_t st) {
     (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(c_rk), tab650, sizeof(double) * 650, 0, hipMemcpyHostToDevice, st);
 }
 
-void launch_deform_k4(const View &d, const int4 *blk16, int nblk, int prepare, hipStream_t st) {
-    if (nblk) hipLaunchKernelGGL(k_deform_k4, dim3(nblk), dim3(64), 0, st, d, blk16, prepare);
+void launch_deform_k4(const View &d, const int4 *blk16, int nblk, int prepare, int np_rows, int max_n, hipStream_t st) {
+    const NpTail np{nblk, np_rows, (max_n + 63) / 64};
+    const int grid = nblk + np_rows * np.chunks;
+    if (grid) hipLaunchKernelGGL(k_deform_k4, dim3(grid), dim3(64), 0, st, d, blk16, prepare, np);
 }
 void launch_deform(const View &d, const int4 *blk_light, int n_light, const int4 *blk_damped, int n_damped,
                    int prepare, hipStream_t st) {
@@ -1757,13 +1779,20 @@ void launch_main_wpb(const View &d, const int2 *work, int nwork, int max_n, bool
 // workgroup still fits on the GPU at once (the ordered sums are serial, so
 // extra wavefronts only pay while workgroups would otherwise be idle
 // CUs; a large batch is faster with one wavefront per body).
-void launch_deform_wpb(const View &d, const int2 *work, int nwork, int max_n, int prepare, hipStream_t st) {
-    if (!nwork) return;
+void launch_deform_wpb(const View &d, const int2 *work, int nwork, int max_n, int prepare, int np_rows,
+                       hipStream_t st) {
+    if (!nwork && !np_rows) return;
     const int chunks = (max_n + 63) / 64;
-    if (chunks >= 8 && nwork <= 512) hipLaunchKernelGGL(k_deform_wpb<8>, dim3(nwork), dim3(512), 0, st, d, work, prepare);
-    else if (chunks >= 4 && nwork <= 1280) hipLaunchKernelGGL(k_deform_wpb<4>, dim3(nwork), dim3(256), 0, st, d, work, prepare);
-    else if (chunks >= 2 && nwork <= 2816) hipLaunchKernelGGL(k_deform_wpb<2>, dim3(nwork), dim3(128), 0, st, d, work, prepare);
-    else hipLaunchKernelGGL(k_deform_wpb<1>, dim3(nwork), dim3(64), 0, st, d, work, prepare);
+    int W = 1;
+    if (chunks >= 8 && nwork <= 512) W = 8;
+    else if (chunks >= 4 && nwork <= 1280) W = 4;
+    else if (chunks >= 2 && nwork <= 2816) W = 2;
+    const NpTail np{nwork, np_rows, (max_n + 64 * W - 1) / (64 * W)};
+    const dim3 grid(nwork + np_rows * np.chunks), block(64 * W);
+    if (W == 8) hipLaunchKernelGGL(k_deform_wpb<8>, grid, block, 0, st, d, work, prepare, np);
+    else if (W == 4) hipLaunchKernelGGL(k_deform_wpb<4>, grid, block, 0, st, d, work, prepare, np);
+    else if (W == 2) hipLaunchKernelGGL(k_deform_wpb<2>, grid, block, 0, st, d, work, prepare, np);
+    else hipLaunchKernelGGL(k_deform_wpb<1>, grid, block, 0, st, d, work, prepare, np);
 }
 void launch_main_serial(const View &d, const int2 *blk, int nblk, hipStream_t st) {
     if (nblk) hipLaunchKernelGGL(k_main_serial<kBS>, dim3(nblk), dim3(kBS), 0, st, d, blk);
@@ -1771,9 +1800,10 @@ void launch_main_serial(const View &d, const int2 *blk, int nblk, hipStream_t st
 void launch_np_field(const View &d, int nrows, int max_n, hipStream_t st) {
     if (nrows) hipLaunchKernelGGL(k_np_field, dim3((max_n + 63) / 64, nrows), dim3(64), 0, st, d, nrows);
 }
-void launch_np(const View &d, int nrows, int max_n, int maxT, const int2 *recv_blk, int nrecv, hipStream_t st) {
+void launch_np(const View &d, int nrows, int max_n, int maxT, const int2 *recv_blk, int nrecv, bool with_field,
+               hipStream_t st) {
     if (!nrows) return;
-    hipLaunchKernelGGL(k_np_field, dim3((max_n + 63) / 64, nrows), dim3(64), 0, st, d, nrows);
+    if (with_field) hipLaunchKernelGGL(k_np_field, dim3((max_n + 63) / 64, nrows), dim3(64), 0, st, d, nrows);
     const int nthr = nrows * maxT;
     hipLaunchKernelGGL(k_np_item, dim3((nthr + 63) / 64), dim3(64), 0, st, d, nrows, maxT);
     if (nrecv) hipLaunchKernelGGL(k_np_recv, dim3(nrecv), dim3(kBS), 0, st, d, recv_blk, maxT);

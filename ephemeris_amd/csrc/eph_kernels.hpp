@@ -58,7 +58,7 @@ void launch_deform(const View &d, const int4 *blk_light, int n_light, const int4
 void launch_main_pool(const View &d, const int2 *blk, int nblk, hipStream_t st);
 // deform of bodies without constant-time lag, four lanes per body;
 // blk16 = (system, first slot, count <= 16, 0) in the class-sorted slots
-void launch_deform_k4(const View &d, const int4 *blk16, int nblk, int prepare, hipStream_t st);
+void launch_deform_k4(const View &d, const int4 *blk16, int nblk, int prepare, int np_rows, int max_n, hipStream_t st);
 // four lanes per body; blk16 = (system, first body) of every 16 bodies
 void launch_main_k4(const View &d, const int2 *blk16, int nblk, hipStream_t st);
 void launch_main_serial(const View &d, const int2 *blk, int nblk, hipStream_t st);
@@ -71,8 +71,12 @@ void launch_np_field(const View &d, int nrows, int max_n, hipStream_t st);
 constexpr int kNpFusedMaxN_h = 512, kNpFusedMaxItems_h = 64;
 // one workgroup of up to 8 wavefronts per body (partner chunks evaluated in
 // parallel, summed in order); max_n = largest system of the batch
-void launch_deform_wpb(const View &d, const int2 *work, int nwork, int max_n, int prepare, hipStream_t st);
-void launch_np(const View &d, int nrows, int max_n, int maxT, const int2 *recv_blk, int nrecv, hipStream_t st);
+// np_rows > 0: the launch also evaluates the non-point item fields
+// (launch_np_field) in extra workgroups
+void launch_deform_wpb(const View &d, const int2 *work, int nwork, int max_n, int prepare, int np_rows,
+                       hipStream_t st);
+void launch_np(const View &d, int nrows, int max_n, int maxT, const int2 *recv_blk, int nrecv, bool with_field,
+               hipStream_t st);
 // istep: optional device step counter (graph replays); launch_step_count adds 1
 void launch_stage(const View &d, int k, double dt, double dt_k, double tadd, const int64_t *istep, hipStream_t st);
 void launch_step_count(int64_t *istep, hipStream_t st);

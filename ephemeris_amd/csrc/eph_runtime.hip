@@ -269,22 +269,28 @@ int Batch::accel(int mode, bool prepare_only) {
     const View v = view(mode);
     const Plan &pl = plan[mode];
     const bool wpb = use_wpb(mode);
+    // the non-point item fields depend only on the stage's positions and
+    // frames: they ride along in extra workgroups of the deform launch
+    const int np_rows = prepare_only ? 0 : pl.nrows;
+    bool field_done = false;
     {
         Scope sc(profiler, 1, stream);
         if (wpb) {
-            launch_deform_wpb(v, d_work_all, n_work_all, max_sys_n, prepare_only ? 0 : 1, stream);
+            launch_deform_wpb(v, d_work_all, n_work_all, max_sys_n, prepare_only ? 0 : 1, np_rows, stream);
+            field_done = true;
         } else if (variant == 1) {
             launch_deform(v, d_blk_light, n_blk_light, d_blk_damped, n_blk_damped, prepare_only ? 0 : 1, stream);
         } else {
             // constant-time-lag bodies re-sum their partners once per spin pass:
-            // one wavefront each, the rest one thread each
-            launch_deform_wpb(v, d_work_damped, n_work_damped, max_sys_n, prepare_only ? 0 : 1, stream);
-            launch_deform_k4(v, d_blk_light16, n_blk_light16, prepare_only ? 0 : 1, stream);
+            // one wavefront each; the rest four lanes each
+            launch_deform_wpb(v, d_work_damped, n_work_damped, max_sys_n, prepare_only ? 0 : 1, 0, stream);
+            launch_deform_k4(v, d_blk_light16, n_blk_light16, prepare_only ? 0 : 1, np_rows, pl.max_n, stream);
+            field_done = true;
         }
     }
     if (prepare_only) return EPH_OK;
     const bool fused_np = wpb && pl.np_fusable && variant != 3;
-    if (fused_np && pl.nrows) {
+    if (fused_np && pl.nrows && !field_done) {
         Scope sc(profiler, 4, stream);
         launch_np_field(v, pl.nrows, pl.max_n, stream);
     }
@@ -300,7 +306,7 @@ int Batch::accel(int mode, bool prepare_only) {
     }
     if (pl.nrows && !fused_np) {
         Scope sc(profiler, 4, stream);
-        launch_np(v, pl.nrows, pl.max_n, pl.maxT, pl.d_blk_recv, pl.n_recv, stream);
+        launch_np(v, pl.nrows, pl.max_n, pl.maxT, pl.d_blk_recv, pl.n_recv, !field_done, stream);
     }
     return EPH_OK;
 }
