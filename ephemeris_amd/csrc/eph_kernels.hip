@@ -38,12 +38,8 @@ struct DevSin {
 struct Acc {
     double *F;
     int64_t P;
-    int shift;                           // 0, or F_BANK1 for the second copy of the state fields
-    __device__ Acc(double *F_, int64_t P_, int bank = 0) : F(F_), P(P_), shift(bank ? (int)F_BANK1 : 0) {}
-    explicit __device__ Acc(const View &d) : Acc(d.F, d.nbp, d.bank) {}
-    __device__ int64_t at(int f, int g) const { return (int64_t)(f < kStateFields ? f + shift : f) * P + g; }
-    __device__ double ld(int f, int g) const { return F[at(f, g)]; }
-    __device__ void st(int f, int g, double v) const { F[at(f, g)] = v; }
+    __device__ double ld(int f, int g) const { return F[(int64_t)f * P + g]; }
+    __device__ void st(int f, int g, double v) const { F[(int64_t)f * P + g] = v; }
     __device__ v3 ldv(int f, int g) const { return v3{ld(f, g), ld(f + 1, g), ld(f + 2, g)}; }
     __device__ void stv(int f, int g, v3 a) const { st(f, g, a.x); st(f + 1, g, a.y); st(f + 2, g, a.z); }
     __device__ m3 ldm(int f, int g) const { return m3{ldv(f, g), ldv(f + 3, g), ldv(f + 6, g)}; }
@@ -55,9 +51,6 @@ struct Acc {
     __device__ ddm3 lddm(int f, int g) const { return ddm3{lddv(f, g), lddv(f + 6, g), lddv(f + 12, g)}; }
     __device__ void stddm(int f, int g, const ddm3 &a) const { stddv(f, g, a.x); stddv(f + 6, g, a.y); stddv(f + 12, g, a.z); }
 };
-
-__device__ void stage_body(const View &d, const Acc &A, const Acc &W, int g, int k, double dt, double dt_k,
-                           double tadd);
 
 // ---------------------------------------------------------------------------
 // geopotential::sum (src/physics/geopotential.impl:64-144)
@@ -294,7 +287,7 @@ __device__ __forceinline__ bool np_tail(const View &d, const NpTail &np) {
     if ((int)blockIdx.x < np.ndef) return false;
     const int x = (int)blockIdx.x - np.ndef;
     const int row = x / np.chunks, c = x % np.chunks;
-    if (row < np.nrows) np_field_one(d, Acc(d), row, c * (int)blockDim.x + (int)threadIdx.x);
+    if (row < np.nrows) np_field_one(d, Acc{d.F, d.nbp}, row, c * (int)blockDim.x + (int)threadIdx.x);
     return true;
 }
 
@@ -377,7 +370,7 @@ __device__ __forceinline__ void deform_write(const Acc &A, int g, int prepare, c
 // blk[b] = {system, first slot, slot count}; slots map to bodies via d.perm.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_deform_light(View d, const int4 *__restrict__ blk, int prepare) {
-    const Acc A(d);
+    const Acc A{d.F, d.nbp};
     const int4 bk = blk[blockIdx.x];
     const SysMeta sm = d.sys[bk.x];
     const int n = sm.n, off = sm.off;
@@ -438,7 +431,7 @@ __global__ __launch_bounds__(BS) void k_deform_light(View d, const int4 *__restr
 constexpr int kDefK4Pitch = 13;
 __global__ __launch_bounds__(64) void k_deform_k4(View d, const int4 *__restrict__ blk, int prepare, NpTail np) {
     if (np_tail(d, np)) return;
-    const Acc A(d);
+    const Acc A{d.F, d.nbp};
     const int4 bk = blk[blockIdx.x];
     const SysMeta sm = d.sys[bk.x];
     const int n = sm.n, off = sm.off;
@@ -527,7 +520,7 @@ __global__ __launch_bounds__(64) void k_deform_k4(View d, const int4 *__restrict
 // (DAMPED_TIDAL_DEFORMATION_MATRIX, mass.impl:1-23).
 template <int BS>
 __global__ __launch_bounds__(BS) void k_deform_damped(View d, const int4 *__restrict__ blk, int prepare) {
-    const Acc A(d);
+    const Acc A{d.F, d.nbp};
     const int4 bk = blk[blockIdx.x];
     const SysMeta sm = d.sys[bk.x];
     const int n = sm.n, off = sm.off;
@@ -719,7 +712,7 @@ struct MainAcc {
 
 template <int BS>
 __global__ __launch_bounds__(BS) void k_main_pool(View d, const int2 *__restrict__ blk) {
-    const Acc A(d);
+    const Acc A{d.F, d.nbp};
     const int s = blk[blockIdx.x].x, i0 = blk[blockIdx.x].y;
     const SysMeta sm = d.sys[s];
     const int n = sm.n, off = sm.off;
@@ -770,7 +763,7 @@ __global__ __launch_bounds__(BS) void k_main_pool(View d, const int2 *__restrict
 constexpr int kK4Pitch = 25;
 
 __global__ __launch_bounds__(64) void k_main_k4(View d, const int2 *__restrict__ blk) {
-    const Acc A(d);
+    const Acc A{d.F, d.nbp};
     const int s = blk[blockIdx.x].x, i0 = blk[blockIdx.x].y;   // bodies i0 .. i0 + 15
     const SysMeta sm = d.sys[s];
     const int n = sm.n, off = sm.off;
@@ -973,9 +966,9 @@ __device__ __forceinline__ void wpb_nonpoint(const View &d, const Acc &A, const 
 }
 
 template <bool NP, int W>
-__global__ __launch_bounds__(64 * W) void k_main_wpb(View d, const int2 *__restrict__ work, Fold fold) {
+__global__ __launch_bounds__(64 * W) void k_main_wpb(View d, const int2 *__restrict__ work) {
     static_assert(!NP || W * kWpbRows * kWpbPitch >= 6 * kNpFusedMaxN, "term tiles double as non-point scratch");
-    const Acc A(d);
+    const Acc A{d.F, d.nbp};
     const int s = work[blockIdx.x].x, il = work[blockIdx.x].y;
     const SysMeta sm = d.sys[s];
     const int n = sm.n, off = sm.off, gi = off + il;
@@ -1046,20 +1039,14 @@ __global__ __launch_bounds__(64 * W) void k_main_wpb(View d, const int2 *__restr
         if (threadIdx.x < 3) gsh[threadIdx.x] = acc;
         __syncthreads();
         wpb_nonpoint(d, A, sm, il, gi, acc, v3{gsh[0], gsh[1], gsh[2]}, term);
-    } else if (wv == 0) {
-        if (lane >= 3 && lane < 6) A.st(F_DACC + lane - 3, gi, acc);
-        else if (lane >= 6 && lane < 9) A.st(F_DTQ + lane - 6, gi, acc);
-        if (sm.nitem == 0 && lane == 11) {                 // FINALIZE_RELATIVITY
-            A.st(F_PHI, gi, I.phi * kC2);
-            A.stv(F_NACC, gi, A.ldv(F_NACCC, gi) * kC2);
-        }
+        return;
     }
-    if (fold.k) {                                          // the next stage's update of this body
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const double tadd = fold.istep ? fold.dt * (fold.dt_k + (double)*fold.istep) : fold.tadd;
-            stage_body(d, A, Acc(d.F, d.nbp, 1 - d.bank), gi, fold.k, fold.dt, fold.dt_k, tadd);
-        }
+    if (wv != 0) return;
+    if (lane >= 3 && lane < 6) A.st(F_DACC + lane - 3, gi, acc);
+    else if (lane >= 6 && lane < 9) A.st(F_DTQ + lane - 6, gi, acc);
+    if (sm.nitem == 0 && lane == 11) {                     // FINALIZE_RELATIVITY
+        A.st(F_PHI, gi, I.phi * kC2);
+        A.stv(F_NACC, gi, A.ldv(F_NACCC, gi) * kC2);
     }
 }
 
@@ -1073,7 +1060,7 @@ constexpr int kWpbDefPitch = 65;
 template <int W>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 1 ? 3 : 1))) void k_deform_wpb(View d, const int2 *__restrict__ work, int prepare, NpTail np) {
     if (np_tail(d, np)) return;
-    const Acc A(d);
+    const Acc A{d.F, d.nbp};
     const int s = work[blockIdx.x].x, il = work[blockIdx.x].y;
     const SysMeta sm = d.sys[s];
     const int n = sm.n, off = sm.off, gi = off + il;
@@ -1224,7 +1211,7 @@ __device__ __forceinline__ void serial_reaction(const View &d, const Acc &A, int
 
 template <int BS>
 __global__ __launch_bounds__(BS) void k_main_serial(View d, const int2 *__restrict__ blk) {
-    const Acc A(d);
+    const Acc A{d.F, d.nbp};
     const int s = blk[blockIdx.x].x, i0 = blk[blockIdx.x].y;
     const SysMeta sm = d.sys[s];
     const int n = sm.n, off = sm.off;
@@ -1286,7 +1273,7 @@ __global__ __launch_bounds__(BS) void k_main_serial(View d, const int2 *__restri
 // ---------------------------------------------------------------------------
 __global__ void k_np_field(View d, int nrows) {
     // one thread per (item, partner): an = field of the item at r_j - r_p
-    const Acc A(d);
+    const Acc A{d.F, d.nbp};
     const int row = blockIdx.y;
     if (row >= nrows) return;
     np_field_one(d, A, row, blockIdx.x * blockDim.x + threadIdx.x);
@@ -1295,7 +1282,7 @@ __global__ void k_np_field(View d, int nrows) {
 __global__ void k_np_item(View d, int nrows, int maxT) {
     // one thread per (item, round k): the item's partial sums over the slice
     // it visits in round k (tpmi of accel_nonpoint)
-    const Acc A(d);
+    const Acc A{d.F, d.nbp};
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = row % maxT, it_i = row / maxT;
     if (it_i >= nrows) return;
@@ -1325,7 +1312,7 @@ __global__ void k_np_item(View d, int nrows, int maxT) {
 __global__ void k_np_recv(View d, const int2 *__restrict__ blk, int maxT) {
     // one thread per body of a system with items: receiver terms in round
     // order, then the body's own item sums, finalisation and ring correction
-    const Acc A(d);
+    const Acc A{d.F, d.nbp};
     const int s = blk[blockIdx.x].x, i0 = blk[blockIdx.x].y;
     const SysMeta sm = d.sys[s];
     const int jl = i0 + (int)threadIdx.x;
@@ -1369,10 +1356,7 @@ __global__ void k_np_recv(View d, const int2 *__restrict__ blk, int maxT) {
 // ---------------------------------------------------------------------------
 // RK12 stage: store f[k-1], combine, update state (RungeKutta.impl:113-186)
 // ---------------------------------------------------------------------------
-// A reads the state of the previous stage; W receives the new state (the
-// same bank for a stand-alone stage launch, the other bank when folded).
-__device__ void stage_body(const View &d, const Acc &A, const Acc &W, int g, int k, double dt, double dt_k,
-                           double tadd) {
+__device__ void stage_body(const View &d, const Acc &A, int g, int k, double dt, double dt_k, double tadd) {
     ddv3 v = A.lddv(F_V, g);
     if (k == 1) {
         A.stddv(F_X0R, g, A.lddv(F_R, g));
@@ -1422,23 +1406,23 @@ __device__ void stage_body(const View &d, const Acc &A, const Acc &W, int g, int
     }
     const ddv3 r0 = A.lddv(F_X0R, g), v0 = A.lddv(F_X0V, g), gl0 = A.lddv(F_X0GL, g);
     const v3 w0 = A.ldv(F_X0W, g);
-    W.stddv(F_R, g, add(r0, scale(mkdd(dt), cv)));
-    W.stddv(F_V, g, add(v0, mkddv(dt * ca)));
+    A.stddv(F_R, g, add(r0, scale(mkdd(dt), cv)));
+    A.stddv(F_V, g, add(v0, mkddv(dt * ca)));
     const v3 w = cw / dt_k;
     ddm3 sm = A.lddm(F_X0S, g);
     const v3 dw = (2.0 / 3) * (w - w0);
     sm = add(sm, rotation_matrix(w - dw, dt * dt_k / 2, DevSin{}) * todbl(sm));
     sm = add(sm, rotation_matrix(w + dw, dt * dt_k / 2, DevSin{}) * todbl(sm));
-    W.stddv(F_GL, g, add(gl0, mkddv(dt * ct)));
+    A.stddv(F_GL, g, add(gl0, mkddv(dt * ct)));
     if (k == 25) {
         sm = add(sm, rotation_matrix(egrad, 1.0 / 72, DevSin{}) * todbl(sm));
         sm = orthogonalized(sm);
     }
-    W.stddm(F_S, g, sm);
-    W.stv(F_W, g, w);
+    A.stddm(F_S, g, sm);
+    A.stv(F_W, g, w);
     const double t = d.t0[d.bsys[g]] + tadd;
-    W.st(F_GM, g, A.ld(F_GM0, g) + A.ld(F_DGM, g) * t);
-    W.st(F_EXJ2, g, A.ld(F_DJ2, g) * t);
+    A.st(F_GM, g, A.ld(F_GM0, g) + A.ld(F_DGM, g) * t);
+    A.st(F_EXJ2, g, A.ld(F_DJ2, g) * t);
 }
 
 // istep != nullptr: the step index comes from device memory (replayed graphs)
@@ -1446,11 +1430,11 @@ __device__ void stage_body(const View &d, const Acc &A, const Acc &W, int g, int
 // same two operations as on the host.
 __global__ __launch_bounds__(64) void k_stage(View d, int k, double dt, double dt_k, double tadd,
                                               const int64_t *__restrict__ istep) {
-    const Acc A(d);
+    const Acc A{d.F, d.nbp};
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= d.nb) return;
     if (istep) tadd = dt * (dt_k + (double)*istep);
-    stage_body(d, A, A, g, k, dt, dt_k, tadd);
+    stage_body(d, A, g, k, dt, dt_k, tadd);
 }
 
 __global__ void k_step_count(int64_t *istep) {
@@ -1575,7 +1559,7 @@ __device__ void main_serial_body_tile(const View &d, const Acc &A, int gi, int i
 // rec[(e*nrec_sys + b)*kSmallMax + t].
 __global__ __launch_bounds__(kSmallMax) void k_small(View d, const int *__restrict__ sys_ids, int n_steps,
                                                      double dt, DtK dtk, int64_t i_step0, JMain *rec, int nrec_sys) {
-    const Acc A(d);
+    const Acc A{d.F, d.nbp};
     const int s = sys_ids[blockIdx.x];
     const SysMeta sm = d.sys[s];
     const int n = sm.n, off = sm.off, t = (int)threadIdx.x;
@@ -1585,7 +1569,7 @@ __global__ __launch_bounds__(kSmallMax) void k_small(View d, const int *__restri
     for (int step = 0; step < n_steps; ++step) {
         for (int k = 1; k <= 25; ++k) {
             const double tadd = dt * (dtk.v[k] + (double)(i_step0 + step));
-            if (valid) stage_body(d, A, A, off + t, k, dt, dtk.v[k], tadd);
+            if (valid) stage_body(d, A, off + t, k, dt, dtk.v[k], tadd);
             __syncthreads();
             if (valid) load_jdeform(A, tdf[t], off + t);
             __syncthreads();
@@ -1604,7 +1588,7 @@ __global__ __launch_bounds__(kSmallMax) void k_small(View d, const int *__restri
 }
 
 __global__ void k_extrema(View d, int invert) {
-    const Acc A(d);
+    const Acc A{d.F, d.nbp};
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= d.nb) return;
     if (invert) A.st(F_MIND, g, 1 / A.ld(F_MIND, g));
@@ -1785,13 +1769,11 @@ void launch_main_pool(const View &d, const int2 *blk, int nblk, hipStream_t st) 
 void launch_main_k4(const View &d, const int2 *blk16, int nblk, hipStream_t st) {
     if (nblk) hipLaunchKernelGGL(k_main_k4, dim3(nblk), dim3(64), 0, st, d, blk16);
 }
-void launch_main_wpb(const View &d, const int2 *work, int nwork, int max_n, bool fused_np, const Fold &fold,
-                     hipStream_t st) {
+void launch_main_wpb(const View &d, const int2 *work, int nwork, int max_n, bool fused_np, hipStream_t st) {
     if (!nwork) return;
-    if (fused_np) hipLaunchKernelGGL((k_main_wpb<true, 2>), dim3(nwork), dim3(128), 0, st, d, work, fold);
-    else if (max_n > 64 && nwork <= 1536)
-        hipLaunchKernelGGL((k_main_wpb<false, 2>), dim3(nwork), dim3(128), 0, st, d, work, fold);
-    else hipLaunchKernelGGL((k_main_wpb<false, 1>), dim3(nwork), dim3(64), 0, st, d, work, fold);
+    if (fused_np) hipLaunchKernelGGL((k_main_wpb<true, 2>), dim3(nwork), dim3(128), 0, st, d, work);
+    else if (max_n > 64 && nwork <= 1536) hipLaunchKernelGGL((k_main_wpb<false, 2>), dim3(nwork), dim3(128), 0, st, d, work);
+    else hipLaunchKernelGGL((k_main_wpb<false, 1>), dim3(nwork), dim3(64), 0, st, d, work);
 }
 // Wavefronts per body: as many as the partner chunks need while every body's
 // workgroup still fits on the GPU at once (the ordered sums are serial, so

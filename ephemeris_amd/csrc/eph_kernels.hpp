@@ -50,13 +50,7 @@ struct View {
     double *an;                  // non-point field per (item, partner), 3 doubles
     double *part;                // per (item, round) partial sums, 6 doubles
     const double *t0;            // per system: t_eph (hi+lo) at the start of integrate
-    int bank = 0;                // state fields read and written: 0 = [0, 41), 1 = [F_BANK1, +41)
 };
-
-// RK12 stage update folded into the tail of a force kernel (k = 0: none):
-// stage k, reading the state of the view's bank and writing the other one.
-// istep != nullptr: the step index comes from device memory.
-struct Fold { int k; double dt, dt_k, tadd; const int64_t *istep; };
 
 void set_rk_table(const double *tab650, hipStream_t st);
 void launch_deform(const View &d, const int4 *blk_light, int n_light, const int4 *blk_damped, int n_damped,
@@ -72,8 +66,7 @@ void launch_main_serial(const View &d, const int2 *blk, int nblk, hipStream_t st
 // fused_np: the receiver and own-item sums of the non-point pass run inside
 // each body's wavefront, after launch_np_field (every thread-pool-order
 // system has at most 512 bodies and 64 non-point items)
-void launch_main_wpb(const View &d, const int2 *work, int nwork, int max_n, bool fused_np, const Fold &fold,
-                     hipStream_t st);
+void launch_main_wpb(const View &d, const int2 *work, int nwork, int max_n, bool fused_np, hipStream_t st);
 void launch_np_field(const View &d, int nrows, int max_n, hipStream_t st);
 constexpr int kNpFusedMaxN_h = 512, kNpFusedMaxItems_h = 64;
 // one workgroup of up to 8 wavefronts per body (partner chunks evaluated in

@@ -45,14 +45,8 @@ enum Field : int {
     F_X0S = 139,        // 18
     F_X0W = 157,        // 3
     F_STAGE = 160,      // 25 stages x 15: v(6 dd), a(3), w(3), dtorque(3)
-    F_COUNT = 160 + 25 * 15,
-    // second copy of the state fields [F_R, F_EXJ2]: an RK12 stage update
-    // folded into the previous stage's force kernel writes the copy that
-    // no force kernel of that stage reads
-    F_BANK1 = F_COUNT,
-    F_ALLOC = F_COUNT + 41
+    F_COUNT = 160 + 25 * 15
 };
-constexpr int kStateFields = 41;     // [0, 41): rewritten by every stage update
 constexpr int kStageFields = 15;
 
 // Non-point force items of one system (geopotential or ring of one body).

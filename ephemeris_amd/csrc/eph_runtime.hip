@@ -264,20 +264,9 @@ bool Batch::use_wpb(int mode) const {
     return plan[mode].n_work_pool > 0 && plan[mode].n_work_pool <= kWpbAutoBodies;
 }
 
-// The next stage's update can ride at the tail of the force kernel when
-// that kernel is the last of the force evaluation for every body: one
-// wavefront per body with the non-point pass fused, no single-thread-order
-// system in the batch.
-bool Batch::can_fold(int mode) const {
-    const Plan &pl = plan[mode];
-    return use_graph_fold && use_wpb(mode) && pl.np_fusable && eff_variant() != 3 && pl.n_serial == 0 &&
-           pl.n_pool > 0;
-}
-
-int Batch::accel(int mode, bool prepare_only, int bank, const Fold *fold) {
+int Batch::accel(int mode, bool prepare_only) {
     const int variant = eff_variant();
-    View v = view(mode);
-    v.bank = bank;
+    const View v = view(mode);
     const Plan &pl = plan[mode];
     const bool wpb = use_wpb(mode);
     // the non-point item fields depend only on the stage's positions and
@@ -307,8 +296,7 @@ int Batch::accel(int mode, bool prepare_only, int bank, const Fold *fold) {
     }
     if (pl.n_pool) {
         Scope sc(profiler, 2, stream);
-        const Fold none{0, 0.0, 0.0, 0.0, nullptr};
-        if (wpb) launch_main_wpb(v, pl.d_work_pool, pl.n_work_pool, pl.max_n, fused_np, fold ? *fold : none, stream);
+        if (wpb) launch_main_wpb(v, pl.d_work_pool, pl.n_work_pool, pl.max_n, fused_np, stream);
         else if (variant == 1) launch_main_pool(v, pl.d_blk_pool, pl.n_pool, stream);
         else launch_main_k4(v, pl.d_blk16, pl.n_blk16, stream);
     }
@@ -392,11 +380,9 @@ int Batch::step_graph(double dt, int mode) {
     profiler.enabled = false;                          // no event records inside the capture
     HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     int rc = EPH_OK;
-    const bool fold = can_fold(mode);
     for (int k = 1; k <= 25 && !rc; ++k) {
-        if (k == 1 || !fold) launch_stage(v, k, dt, dtk.v[k], 0.0, d_istep, stream);
-        const Fold f{fold && k < 25 ? k + 1 : 0, dt, k < 25 ? dtk.v[k + 1] : 0.0, 0.0, d_istep};
-        rc = accel(mode, false, fold ? (k + 1) & 1 : 0, &f);
+        launch_stage(v, k, dt, dtk.v[k], 0.0, d_istep, stream);
+        rc = accel(mode, false);
     }
     launch_step_count(d_istep, stream);
     hipGraph_t g = nullptr;
@@ -490,17 +476,19 @@ int Batch::integrate(double dt, int64_t n_step, int mode) {
         for (int64_t s = 0; s < n_sys; ++s) t_eph[s] = host_add(t_eph[s], mkdd(dt * (double)n_step));
         return EPH_OK;
     }
-    const DtK dtk = stage_dtk();                            // RungeKutta.impl:148-152
-    const bool fold = can_fold(mode);
     for (int64_t i_step = 0; i_step < n_step; ++i_step) {
         for (int k = 1; k <= 25; ++k) {
-            if (k == 1 || !fold) {
-                Scope sc(profiler, 0, stream);
-                launch_stage(v, k, dt, dtk.v[k], dt * (dtk.v[k] + (double)i_step), nullptr, stream);
+            double dt_k = 0;                                   // RungeKutta.impl:148-152
+            for (int j = 0; j < k; ++j) {
+                const double *c = kRK[k * (k - 1) / 2 + j];
+                if (c[0] != 0) dt_k += c[0] + c[1];
             }
-            Fold f{0, 0.0, 0.0, 0.0, nullptr};
-            if (fold && k < 25) f = Fold{k + 1, dt, dtk.v[k + 1], dt * (dtk.v[k + 1] + (double)i_step), nullptr};
-            int rc = accel(mode, false, fold ? (k + 1) & 1 : 0, &f);
+            const double tadd = dt * (dt_k + (double)i_step);
+            {
+                Scope sc(profiler, 0, stream);
+                launch_stage(v, k, dt, dt_k, tadd, nullptr, stream);
+            }
+            int rc = accel(mode, false);
             if (rc) return rc;
         }
         if (profiler.enabled && (i_step & 15) == 15) profiler.collect();
@@ -654,7 +642,7 @@ int create_batch(int64_t n_sys, const int64_t *sys_n, const eph_mass *masses,
     auto bail = [&](int code) { delete b; return code; };
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(EPH_EDEVICE, "hipStreamCreate failed"));
-    size_t bytes = sizeof(double) * (size_t)F_ALLOC * b->nbp;
+    size_t bytes = sizeof(double) * (size_t)F_COUNT * b->nbp;
     if (hipMalloc(&b->F, bytes) != hipSuccess) return bail(fail(EPH_ENOMEM, "hipMalloc of the state failed"));
     b->dev_allocs.push_back(b->F);
     if (hipMemset(b->F, 0, bytes) != hipSuccess) return bail(fail(EPH_EDEVICE, "hipMemset failed"));

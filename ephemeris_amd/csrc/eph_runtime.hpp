@@ -104,11 +104,7 @@ struct Batch {
     int build_plan(int mode);
     int build_class_tables(const eph_mass *masses);
     View view(int mode) const;
-    // bank: state copy the force evaluation reads; fold: next stage update
-    // carried by the force kernel (see can_fold)
-    int accel(int mode, bool deform_only, int bank = 0, const Fold *fold = nullptr);
-    bool can_fold(int mode) const;
-    bool use_graph_fold = true;
+    int accel(int mode, bool deform_only);
     int integrate(double dt, int64_t n_step, int mode);
     // persistent path: all systems serial and N <= kSmallMax; optional record
     int integrate_small(double dt, int64_t n_step, JMain *rec, int64_t rec_steps,

@@ -219,7 +219,7 @@ int Server::admit(int kind, std::vector<SJob *> &fresh) {
             used_slots += is_ens ? (int64_t)it->second->sys_n.size() : it->second->units;
     }
     const int64_t max_slots = is_ens ? cfg.max_systems : cfg.max_particles;
-    const int64_t resident = is_ens ? (int64_t)F_ALLOC * 8 * (ens ? ens->nb : 0) : (int64_t)P_COUNT_HOST * 8 * npp;
+    const int64_t resident = is_ens ? (int64_t)F_COUNT * 8 * (ens ? ens->nb : 0) : (int64_t)P_COUNT_HOST * 8 * npp;
     int64_t reserve = 0;                   // a re-formed batch holds old and new copies at once
     for (auto qi = queue.begin(); qi != queue.end();) {
         SJob *j = jobs[*qi].get();
@@ -586,7 +586,7 @@ int eph_server_submit_ensemble(eph_server *srv, int64_t n_sys, const int64_t *sy
     }
     if (!why.empty()) return fail(EPH_ESANITY, "job rejected, sanity check failed: " + why);
     j->units = nb;
-    j->bytes = (int64_t)eph::F_ALLOC * 8 * (nb + 64) + eph::kSysOverhead * n_sys;
+    j->bytes = (int64_t)eph::F_COUNT * 8 * (nb + 64) + eph::kSysOverhead * n_sys;
     return s->submit(std::move(j), job);
 }
 

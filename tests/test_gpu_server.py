@@ -111,7 +111,7 @@ def test_admission_by_memory_budget_and_worker(setup):
     cfg, members, r, v = setup
     print("\n[case] server budget: jobs queue behind a small memory budget, worker thread", flush=True)
     one = _job(members, [0])
-    need = 576 * 8 * (len(one[0]) + 64) + 16384
+    need = 535 * 8 * (len(one[0]) + 64) + 16384
     srv = Server(_field(cfg), ens_dt=ENS_DT, ens_quantum=ENS_Q, part_dt=PART_DT, part_quantum=PART_Q,
                  part_max_steps=PART_MAX, mem_budget=int(need * 1.5))
     ids = [srv.submit_ensemble(*one, n_steps=6) for _ in range(3)]
