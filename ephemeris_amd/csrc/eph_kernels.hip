@@ -429,7 +429,8 @@ __global__ __launch_bounds__(BS) void k_deform_light(View d, const int4 *__restr
 // subtracted terms stored negated.  The self pair and partners past the end
 // add +0, an exact identity.
 constexpr int kDefK4Pitch = 13;
-__global__ __launch_bounds__(64) void k_deform_k4(View d, const int4 *__restrict__ blk, int prepare) {
+__global__ __launch_bounds__(64) void k_deform_k4(View d, const int4 *__restrict__ blk, int prepare, NpTail np) {
+    if (np_tail(d, np)) return;
     const Acc A{d.F, d.nbp};
     const int4 bk = blk[blockIdx.x];
     const SysMeta sm = d.sys[bk.x];
@@ -1752,8 +1753,10 @@ void set_rk_table(const double *tab650, hipStream_t st) {
     (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(c_rk), tab650, sizeof(double) * 650, 0, hipMemcpyHostToDevice, st);
 }
 
-void launch_deform_k4(const View &d, const int4 *blk16, int nblk, int prepare, hipStream_t st) {
-    if (nblk) hipLaunchKernelGGL(k_deform_k4, dim3(nblk), dim3(64), 0, st, d, blk16, prepare);
+void launch_deform_k4(const View &d, const int4 *blk16, int nblk, int prepare, int np_rows, int max_n, hipStream_t st) {
+    const NpTail np{nblk, np_rows, (max_n + 63) / 64};
+    const int grid = nblk + np_rows * np.chunks;
+    if (grid) hipLaunchKernelGGL(k_deform_k4, dim3(grid), dim3(64), 0, st, d, blk16, prepare, np);
 }
 void launch_deform(const View &d, const int4 *blk_light, int n_light, const int4 *blk_damped, int n_damped,
                    int prepare, hipStream_t st) {

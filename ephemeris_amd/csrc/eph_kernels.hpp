@@ -58,7 +58,7 @@ void launch_deform(const View &d, const int4 *blk_light, int n_light, const int4
 void launch_main_pool(const View &d, const int2 *blk, int nblk, hipStream_t st);
 // deform of bodies without constant-time lag, four lanes per body;
 // blk16 = (system, first slot, count <= 16, 0) in the class-sorted slots
-void launch_deform_k4(const View &d, const int4 *blk16, int nblk, int prepare, hipStream_t st);
+void launch_deform_k4(const View &d, const int4 *blk16, int nblk, int prepare, int np_rows, int max_n, hipStream_t st);
 // four lanes per body; blk16 = (system, first body) of every 16 bodies
 void launch_main_k4(const View &d, const int2 *blk16, int nblk, hipStream_t st);
 void launch_main_serial(const View &d, const int2 *blk, int nblk, hipStream_t st);

@@ -128,9 +128,6 @@ Batch::~Batch() {
     profiler.collect();
     for (void *p : dev_allocs) (void)hipFree(p);
     for (void *p : class_allocs) (void)hipFree(p);
-    if (ev_fork) (void)hipEventDestroy(ev_fork);
-    if (ev_join) (void)hipEventDestroy(ev_join);
-    if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -284,21 +281,10 @@ int Batch::accel(int mode, bool prepare_only) {
         } else if (variant == 1) {
             launch_deform(v, d_blk_light, n_blk_light, d_blk_damped, n_blk_damped, prepare_only ? 0 : 1, stream);
         } else {
-            // the item fields on the side stream, beside the deform kernels
-            if (np_rows) {
-                HIP_CHECK(hipEventRecord(ev_fork, stream));
-                HIP_CHECK(hipStreamWaitEvent(side, ev_fork, 0));
-                Scope sn(profiler, 4, side);
-                launch_np_field(v, np_rows, pl.max_n, side);
-            }
             // constant-time-lag bodies re-sum their partners once per spin pass:
             // one wavefront each; the rest four lanes each
             launch_deform_wpb(v, d_work_damped, n_work_damped, max_sys_n, prepare_only ? 0 : 1, 0, stream);
-            launch_deform_k4(v, d_blk_light16, n_blk_light16, prepare_only ? 0 : 1, stream);
-            if (np_rows) {
-                HIP_CHECK(hipEventRecord(ev_join, side));
-                join_pending = true;
-            }
+            launch_deform_k4(v, d_blk_light16, n_blk_light16, prepare_only ? 0 : 1, np_rows, pl.max_n, stream);
             field_done = true;
         }
     }
@@ -317,10 +303,6 @@ int Batch::accel(int mode, bool prepare_only) {
     if (pl.n_serial) {
         Scope sc(profiler, 3, stream);
         launch_main_serial(v, pl.d_blk_serial, pl.n_serial, stream);
-    }
-    if (join_pending) {                                     // item fields ready before their sums
-        HIP_CHECK(hipStreamWaitEvent(stream, ev_join, 0));
-        join_pending = false;
     }
     if (pl.nrows && !fused_np) {
         Scope sc(profiler, 4, stream);
@@ -658,10 +640,7 @@ int create_batch(int64_t n_sys, const int64_t *sys_n, const eph_mass *masses,
     }
     int rc;
     auto bail = [&](int code) { delete b; return code; };
-    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess)
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(EPH_EDEVICE, "hipStreamCreate failed"));
     size_t bytes = sizeof(double) * (size_t)F_COUNT * b->nbp;
     if (hipMalloc(&b->F, bytes) != hipSuccess) return bail(fail(EPH_ENOMEM, "hipMalloc of the state failed"));

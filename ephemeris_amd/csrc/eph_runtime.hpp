@@ -57,9 +57,6 @@ struct Plan {
 struct Batch {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipStream_t side = nullptr;          // item fields beside the deform kernels (fork/join by events)
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    bool join_pending = false;
     int64_t n_sys = 0, nb = 0, nbp = 0;
     int max_sys_n = 0;
     std::vector<int64_t> sys_n, sys_off;
