@@ -199,8 +199,8 @@ def main():
     ap.add_argument("--particles", type=int, default=1000000)
     ap.add_argument("--particle-steps", type=int, default=4)
     ap.add_argument("--no-particles", action="store_true")
-    ap.add_argument("--serve-jobs", type=int, default=30)
-    ap.add_argument("--serve-rate", type=float, default=0.5, help="job arrivals per second (Poisson)")
+    ap.add_argument("--serve-jobs", type=int, default=12)
+    ap.add_argument("--serve-rate", type=float, default=1.0, help="job arrivals per second (Poisson)")
     ap.add_argument("--serve-seed", type=int, default=20254)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
