@@ -36,14 +36,14 @@ FLOP_DEFORM_PAIR = {"nk2": 54, "ndelay": 89, "damped": 217}
 PROFILE = ROOT / "profiles" / "kernels_full504_ensemble256_particles1e6.md"
 
 
-def profiled_traffic(kernel_prefix):
-    """Calibrated HBM bytes per dispatch (read + write) of a kernel, from the
-    committed rocprofv3 PMC summary (None if the kernel is not listed)."""
+def profiled_traffic(kernel_prefix, grid):
+    """Calibrated HBM bytes per dispatch (read + write) of a kernel at one grid
+    size, from the committed rocprofv3 PMC summary (None if not listed)."""
     try:
         for line in PROFILE.read_text().splitlines():
             cols = [c.strip() for c in line.strip().strip("|").split("|")]
-            if len(cols) == 5 and cols[0].strip("`").startswith(kernel_prefix):
-                return (float(cols[3]) + float(cols[4])) * 1e6
+            if len(cols) == 6 and cols[0].strip("`").startswith(kernel_prefix) and cols[1] == str(grid):
+                return (float(cols[4]) + float(cols[5])) * 1e6
     except (OSError, ValueError):
         pass
     return None
@@ -251,7 +251,7 @@ def main():
                 "dominant_kernel": dominant, "bound": "mfma", "compute": "FP64 vector ALU (same 78.6 TFLOP/s roof; no MFMA used)",
                 "achieved": achieved, "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP64_VECTOR_PEAK_TFLOPS,
-                "traffic": profiled_traffic("void eph::k_main_wpb<true, 2>"), "traffic_unit": "bytes per launch",
+                "traffic": profiled_traffic("void eph::k_main_wpb<true, 2>", n * 128), "traffic_unit": "bytes per launch",
                 "flop_per_launch": flop_main, "avg_launch_ms": main_ms,
                 "kernel_ms": {k: round(v[0], 5) for k, v in prof.items()},
                 "kernel_launches": {k: v[1] for k, v in prof.items()}}
