@@ -857,11 +857,18 @@ constexpr int kWpbZero = kWpbRows - 1;
 // thread-pool order, mass.cpp:147-155): lanes 0-2 gaccel (+G), 3-5 daccel
 // (+own quadrupole, +partner's quadrupole, -LT own spin, +LT partner's spin,
 // +radiation), 6-8 dtorque (+r x dg, -LT torque)
+// Term rows: G 0-2, dg 3-5, r x dg 6-8, gj 9-11, -LT torque 12-14,
+// -LT own spin 15-17, LT partner's spin 18-20, radiation 21-23, zeros 24.
+// With an odd row pitch, the nine rows the accumulating lanes read at one
+// step are distinct modulo 16, so the reads hit distinct LDS banks.
 __device__ __forceinline__ int wpb_main_row(int lane, int k) {
     const int c = lane % 3;
     if (lane < 3) return k == 0 ? c : kWpbZero;
-    if (lane < 6) return 3 + 3 * k + c;
-    if (lane < 9) return k < 2 ? 18 + 3 * k + c : kWpbZero;
+    if (lane < 6) {
+        constexpr int base[kWpbSteps] = {3, 9, 15, 18, 21};     // dg, gj, -lt1, lt3, rad
+        return base[k] + c;
+    }
+    if (lane < 9) return k == 0 ? 6 + c : (k == 1 ? 12 + c : kWpbZero);
     return kWpbZero;
 }
 
@@ -990,10 +997,10 @@ __global__ __launch_bounds__(64 * W) void k_main_wpb(View d, const int2 *__restr
             JMain T;
             load_jmain(A, T, off + jl);
             const PairTerms pt = pair_terms(I, T);
-            const double v[kWpbZero] = {pt.G.x, pt.G.y, pt.G.z, pt.dg.x, pt.dg.y, pt.dg.z, pt.gj.x, pt.gj.y, pt.gj.z,
-                                        -pt.lt1.x, -pt.lt1.y, -pt.lt1.z, pt.lt3.x, pt.lt3.y, pt.lt3.z,
-                                        pt.rad.x, pt.rad.y, pt.rad.z, pt.rxdg.x, pt.rxdg.y, pt.rxdg.z,
-                                        -pt.lt2.x, -pt.lt2.y, -pt.lt2.z};
+            const double v[kWpbZero] = {pt.G.x, pt.G.y, pt.G.z, pt.dg.x, pt.dg.y, pt.dg.z,
+                                        pt.rxdg.x, pt.rxdg.y, pt.rxdg.z, pt.gj.x, pt.gj.y, pt.gj.z,
+                                        -pt.lt2.x, -pt.lt2.y, -pt.lt2.z, -pt.lt1.x, -pt.lt1.y, -pt.lt1.z,
+                                        pt.lt3.x, pt.lt3.y, pt.lt3.z, pt.rad.x, pt.rad.y, pt.rad.z};
 #pragma unroll
             for (int q = 0; q < kWpbZero; ++q) t[q] = v[q];
             mind = checked_max_into(mind, pt.rr);
