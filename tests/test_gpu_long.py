@@ -60,6 +60,7 @@ def test_full504_ten_days():
 @pytest.mark.timeout(1200)
 @pytest.mark.skipif(os.environ.get("EPH_LONG") != "1", reason="one simulated year takes minutes; set EPH_LONG=1")
 def test_full504_one_year():
-    print("\n[case] full504 to one year (105192 steps)", flush=True)
-    rows = _run(105192)
+    upto = int(os.environ.get("EPH_LONG_STEPS", "105192"))
+    print(f"\n[case] full504 to {upto} steps (one year = 105192)", flush=True)
+    rows = _run(upto)
     assert rows and all(dr <= POS_TOL for _, dr in rows), rows
