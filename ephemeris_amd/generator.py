@@ -221,12 +221,33 @@ def _star(rng, tides):
     return s
 
 
-def _planets(rng, star, n=8, tides=False):
+def _hill_spaced(a_sorted, mass_sorted, ecc_sorted, m_central, k_hill=10.0):
+    """True when neighbouring orbits are at least k_hill mutual Hill radii
+    apart beyond their eccentric excursions (no close encounters)."""
+    for q in range(len(a_sorted) - 1):
+        a1, a2 = a_sorted[q], a_sorted[q + 1]
+        rh = ((mass_sorted[q] + mass_sorted[q + 1]) / (3 * m_central)) ** (1 / 3) * (a1 + a2) / 2
+        if a2 * (1 - ecc_sorted[q + 1]) - a1 * (1 + ecc_sorted[q]) < k_hill * rh:
+            return False
+    return True
+
+
+def _planets(rng, star, n=8, tides=False, spaced=False):
     out = []
+    if spaced:
+        # orbits drawn until every neighbouring pair is Hill-stable
+        while True:
+            a_all = np.array([_loguni(rng, 0.39, 30.0) for _ in range(n)]) * AU
+            e_all = rng.uniform(0, 0.1, n)
+            m_all = np.array([_loguni(rng, 3e23, 2e27) for _ in range(n)])
+            o = np.argsort(a_all)
+            if _hill_spaced(a_all[o], m_all[o], e_all[o], M_SUN):
+                break
     for k in range(n):
-        a = _loguni(rng, 0.39, 30.0) * AU
-        e, inc = rng.uniform(0, 0.1), rng.uniform(0, 3.0) * math.pi / 180
-        mass = _loguni(rng, 3e23, 2e27)
+        a = _loguni(rng, 0.39, 30.0) * AU if not spaced else a_all[k]
+        e = rng.uniform(0, 0.1) if not spaced else e_all[k]
+        inc = rng.uniform(0, 3.0) * math.pi / 180
+        mass = _loguni(rng, 3e23, 2e27) if not spaced else m_all[k]
         rho = rng.uniform(700, 5500) if mass > 1e26 else rng.uniform(3000, 5500)
         Om, om, M = rng.uniform(0, 2 * math.pi, 3)
         r, v, nrm = kepler_state(star.GM + G * mass, a, e, inc, Om, om, M)
@@ -244,20 +265,42 @@ def _planets(rng, star, n=8, tides=False):
     return out
 
 
+def _spaced_moon_orbits(rng, planet, ratios, lo, cap, k_hill=10.0):
+    """Semi-major axes from lo outward, each pair of neighbours at least
+    k_hill mutual Hill radii apart (beyond eccentricities <= 0.02); masses are
+    scaled down by ten until the moons fit inside cap."""
+    ratios = np.array(ratios, float)
+    while True:
+        a = [lo * rng.uniform(1.0, 1.3)]
+        for q in range(1, len(ratios)):
+            h = ((ratios[q - 1] + ratios[q]) / 3) ** (1 / 3)
+            a.append(a[-1] * (1 + k_hill * h + 0.05) * rng.uniform(1.0, 1.15))
+        if a[-1] <= cap or ratios.max() < 1e-12:
+            return np.array(a), ratios
+        ratios = ratios / 10
+
+
 def _moons(rng, planet, count, prefix, major_ratio=None, tides=False, rmin=5.0, rmax=60.0,
-           ratio_lo=1e-8, ratio_hi=1e-2, hill_cap=True, star_gm=G * M_SUN):
+           ratio_lo=1e-8, ratio_hi=1e-2, hill_cap=True, star_gm=G * M_SUN, spaced=False):
     out = []
     Rp = planet.R_km * 1000.0
     a_p = getattr(planet, "_a", None)
+    if spaced and count:
+        cap = 0.3 * a_p * (planet.GM / (3 * star_gm)) ** (1 / 3) if a_p else rmax * Rp
+        lo = min(rmin * Rp, 0.1 * cap)
+        ratios = [_loguni(rng, ratio_lo, ratio_hi) for _ in range(count)]
+        rng.shuffle(ratios)
+        a_sp, ratios = _spaced_moon_orbits(rng, planet, ratios, lo, cap)
     for q in range(count):
-        ratio = _loguni(rng, ratio_lo, ratio_hi)
+        ratio = _loguni(rng, ratio_lo, ratio_hi) if not spaced else ratios[q]
         mass = ratio * planet.GM / G
         hi = rmax * Rp
         if hill_cap and a_p:
             hi = min(hi, 0.3 * a_p * (planet.GM / (3 * star_gm)) ** (1 / 3))
         lo = min(rmin * Rp, 0.5 * hi)
-        a = rng.uniform(lo, hi)
-        e, inc = rng.uniform(0, 0.05), rng.uniform(0, 0.1)
+        a = rng.uniform(lo, hi) if not spaced else a_sp[q]
+        e = rng.uniform(0, 0.05) if not spaced else rng.uniform(0, 0.02)
+        inc = rng.uniform(0, 0.1)
         Om, om, M = rng.uniform(0, 2 * math.pi, 3)
         r, v, nrm = kepler_state(planet.GM + G * mass, a, e, inc, Om, om, M)
         m = Body(f"{planet.name} moon {q + 1}", f"{prefix}{q + 1:02d}", G * mass,
@@ -299,7 +342,7 @@ def make_full(seed: int = 20251) -> SystemSpec:
     star = _star(rng, tides=True)
     star.gp, star.gp_terms = 1.0, _zonal_terms(rng, 2e-7)
     star.extra = {"MassRate": -0.3784717910405785, "Luminosity": 3.828e26}
-    planets = _planets(rng, star, 8, tides=True)
+    planets = _planets(rng, star, 8, tides=True, spaced=True)
     for p in planets:
         p.gp, p.gp_terms = 1.0, _zonal_terms(rng, p.J2)
     big = sorted(planets, key=lambda p: -p.GM)
@@ -319,7 +362,7 @@ def make_full(seed: int = 20251) -> SystemSpec:
     for k, p in enumerate(planets):
         bodies.append(p)
         bodies += _moons(rng, p, int(alloc[k]), f"M{k + 1}_", major_ratio=1e-6, tides=True,
-                         rmin=4.0, rmax=80.0)
+                         rmin=4.0, rmax=80.0, ratio_hi=1e-3, spaced=True)
     # dwarf planets: one in the main belt (also one of the 343 asteroids), four TNOs
     dwarfs = []
     dwarf_a = [rng.uniform(2.6, 2.9)] + list(rng.uniform(32, 48, 4))
@@ -337,7 +380,7 @@ def make_full(seed: int = 20251) -> SystemSpec:
     for k, d in enumerate(dwarfs):
         bodies.append(d)
         bodies += _moons(rng, d, dm_alloc[k], f"D{k + 1}_", rmin=5.0, rmax=60.0,
-                         ratio_lo=1e-5, ratio_hi=1e-1)
+                         ratio_lo=1e-5, ratio_hi=1e-1, spaced=True)
     for q in range(342):
         mass = _loguni(rng, 1e15, 1e21)
         a = rng.uniform(2.1, 3.3) * AU
