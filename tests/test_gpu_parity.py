@@ -34,6 +34,27 @@ def _ulp_diff(a, b):
     return np.max(np.abs(a - b) / scale) if a.size else 0.0
 
 
+# Outputs that carry a ring's field: the ring model's atan2/log/sin/cos on
+# the device may differ from glibc in the last ulp, and the ring acts on
+# every body (receiver terms) and on its host (own sums, ring correction).
+RING_FIELDS = tuple(f"{q}_{c}" for q in ("daccel", "dtorque", "jdaccel", "jdtorque") for c in "xyz")
+
+
+def _ring_tolerant_mismatch(m, ref):
+    """Fields that differ from the reference: bit-exact, except that in a
+    system with a ring the ring-carrying outputs may differ by 1e-13 relative."""
+    has_ring = bool((ref["ring"] >= 0).any())
+    bad = {}
+    for f in ACCEL_OUTPUTS:
+        if bits_equal(m[f], ref[f]):
+            continue
+        rel = _ulp_diff(m[f], ref[f])
+        if has_ring and f in RING_FIELDS and rel < 1e-13:
+            continue
+        bad[f] = (int((m[f].view(np.uint64) != ref[f].view(np.uint64)).sum()), rel)
+    return bad
+
+
 @pytest.mark.parametrize("name,tag", [("small25", "serial"), ("full504", "pool"), ("full504", "serial")])
 def test_accel_matches_reference_bitwise(name, tag):
     print(f"\n[case] accel {name} {tag}", flush=True)
@@ -41,19 +62,10 @@ def test_accel_matches_reference_bitwise(name, tag):
     g = _gpu(d)
     g.accel(_lib.SUMMATION_POOL if tag == "pool" else _lib.SUMMATION_SERIAL)
     m, _ = g.download()
-    ref = d["masses_out"]
-    ring_hosts = set(np.nonzero(ref["ring"] >= 0)[0].tolist())
-    bad = {}
-    for f in ACCEL_OUTPUTS:
-        if not bits_equal(m[f], ref[f]):
-            bad[f] = (int((m[f].view(np.uint64) != ref[f].view(np.uint64)).sum()), _ulp_diff(m[f], ref[f]))
-    print("mismatching fields (count, max rel):", bad, flush=True)
-    if not ring_hosts:
-        assert not bad
-    else:
-        # a ring's transcendentals may differ by an ulp; everything else is exact
-        for f, (cnt, rel) in bad.items():
-            assert rel < 1e-13, (f, cnt, rel)
+    exact = sum(bits_equal(m[f], d["masses_out"][f]) for f in ACCEL_OUTPUTS)
+    bad = _ring_tolerant_mismatch(m, d["masses_out"])
+    print(f"{exact}/{len(ACCEL_OUTPUTS)} output fields bit-identical; beyond the ring allowance: {bad}", flush=True)
+    assert not bad
 
 
 def test_deform_matches_reference():
@@ -115,18 +127,23 @@ def test_integrate_is_deterministic_and_chunk_consistent():
     assert bits_equal(ma, mb)
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4])
 @pytest.mark.parametrize("name,tag", [("full504", "pool"), ("small25", "serial")])
-def test_kernel_variants_bitwise(variant, name, tag):
-    """Per-thread and per-wavefront kernels reproduce the reference bits."""
-    print(f"\n[case] variant {variant} accel {name} {tag}", flush=True)
+def test_kernel_variants_bitwise(name, tag):
+    """Every kernel variant gives the same bits; all match the reference
+    (ring hosts to an ulp of their transcendentals)."""
+    print(f"\n[case] variants 1-4 accel {name} {tag}", flush=True)
     d = load_npz(GOLDEN / f"accel_{name}_{tag}.npz")
-    g = _gpu(d)
-    g.set_variant(variant, allow_persistent=False)
-    g.accel(_lib.SUMMATION_POOL if tag == "pool" else _lib.SUMMATION_SERIAL)
-    m, _ = g.download()
-    bad = [f for f in ACCEL_OUTPUTS if not bits_equal(m[f], d["masses_out"][f])]
-    assert not bad, bad
+    outs = {}
+    for variant in (1, 2, 3, 4):
+        g = _gpu(d)
+        g.set_variant(variant, allow_persistent=False)
+        g.accel(_lib.SUMMATION_POOL if tag == "pool" else _lib.SUMMATION_SERIAL)
+        outs[variant], _ = g.download()
+        bad = _ring_tolerant_mismatch(outs[variant], d["masses_out"])
+        assert not bad, (variant, bad)
+    for variant in (2, 3, 4):
+        diff = [f for f in outs[1].dtype.names if not bits_equal(outs[variant][f], outs[1][f])]
+        assert not diff, (variant, diff)
 
 
 @pytest.mark.parametrize("variant,persistent", [(1, False), (2, False), (3, False), (4, False), (0, True)])
