@@ -67,6 +67,9 @@ __device__ v3 gp_sum(const eph_geopotential &gp, double R, v3 r) {
     double cm, sm;
     if (rho2 < 2.2250738585072014e-308) { cm = 1; sm = 0; }
     else { cm = r.x / rho1; sm = r.y / rho1; }
+    // Both loops are unrolled to the maximum degree (the loader rejects
+    // larger ones), so every index of pnm / pn1m is a constant and the
+    // arrays live in registers, not in scratch memory.
     double pnm[EPH_GP_MAX_DEGREE], pn1m[EPH_GP_MAX_DEGREE];
     pnm[0] = 0;
     pn1m[0] = 1;
@@ -74,7 +77,9 @@ __device__ v3 gp_sum(const eph_geopotential &gp, double R, v3 r) {
     double zx = 0, zz = 0;
     v3 t{0, 0, 0};
     const int Nz = (int)gp.Nz, Nt = (int)gp.Nt;
-    for (int n = 2; n <= Nz; ++n) {
+#pragma unroll
+    for (int n = 2; n <= EPH_GP_MAX_DEGREE; ++n) {
+        if (n > Nz) break;
         rbn *= rb1;
         const double *JCSn = gp.c + (n * n - 4);
         pn = ((double)(2 * n - 1) * sphi * pn1 - (double)(n - 1) * pn) / (double)n;
@@ -88,7 +93,9 @@ __device__ v3 gp_sum(const eph_geopotential &gp, double R, v3 r) {
         double m1 = (double)(2 * n - 1) * sphi;
         double m2 = (double)(-n) * sphi;
         double smm = 0, cmm = 0, smm1 = 0, cmm1 = 0;
-        for (int m = 1; m <= n; ++m) {
+#pragma unroll
+        for (int m = 1; m <= EPH_GP_MAX_DEGREE; ++m) {
+            if (m > n) break;
             if (m == n) {
                 pn1m[m - 1] = 0;
                 pnm[m - 1] = (double)(2 * n - 1) * cphi * pnm[m - 2];
@@ -223,7 +230,7 @@ __device__ v3 ring_sum(const eph_ring &rg, double host_R, v3 r) {
 
 // world-frame acceleration (per unit GM of the non-point body p) at offset r
 // from p: geopotential (mass.cpp:186-188) or ring (mass.cpp:200-203)
-__device__ v3 nonpoint_field(const View &d, const Acc &A, int p, int kind, int model, v3 r) {
+__device__ __forceinline__ v3 nonpoint_field_inl(const View &d, const Acc &A, int p, int kind, int model, v3 r) {
     double R = A.ld(F_RAD, p);
     if (kind == 0) {
         m3 fmis = todbl(A.lddm(F_S, p));
@@ -235,6 +242,11 @@ __device__ v3 nonpoint_field(const View &d, const Acc &A, int p, int kind, int m
     m3 fgls{x, cross(z, x), z};
     v3 lr = tolocal(fgls, r);
     return fgls * ring_sum(d.rings[model], R, lr);
+}
+// The same out of line, for the kernels that already need every register
+// (serial order, small systems): they pay a call instead.
+__device__ __noinline__ v3 nonpoint_field_call(const View &d, const Acc &A, int p, int kind, int model, v3 r) {
+    return nonpoint_field_inl(d, A, p, kind, model, r);
 }
 
 
@@ -273,23 +285,60 @@ __device__ __forceinline__ void np_field_one(const View &d, const Acc &A, int ro
     v3 an{0, 0, 0};
     if (gj != it.body) {
         const v3 r = diff(A.lddv(F_R, gj), A.lddv(F_R, it.body));
-        an = nonpoint_field(d, A, it.body, it.kind, it.model, r);
+        an = nonpoint_field_inl(d, A, it.body, it.kind, it.model, r);
     }
     double *o = d.an + 3 * ((int64_t)it.an_off + jl);
     o[0] = an.x; o[1] = an.y; o[2] = an.z;
 }
 
-// Extra workgroups appended to a deform launch: workgroup `x` past the
-// deform ones covers partners [c * blockDim, (c + 1) * blockDim) of item row
-// x / np.chunks.
-struct NpTail { int ndef, nrows, chunks; };
+// Extra workgroups appended to a deform launch: each wavefront past the
+// deform workgroups covers 64 partners of one item row.  Units are taken
+// chunk-major, so the chunks of one row (a ring's are costly) run in
+// different workgroups and on different CUs.
+struct NpTail { int ndef, nrows, chunks; };           // chunks of 64 partners per row
 __device__ __forceinline__ bool np_tail(const View &d, const NpTail &np) {
     if ((int)blockIdx.x < np.ndef) return false;
-    const int x = (int)blockIdx.x - np.ndef;
-    const int row = x / np.chunks, c = x % np.chunks;
-    if (row < np.nrows) np_field_one(d, Acc{d.F, d.nbp}, row, c * (int)blockDim.x + (int)threadIdx.x);
+    if (np.nrows <= 0) return true;
+    const int u = ((int)blockIdx.x - np.ndef) * ((int)blockDim.x >> 6) + ((int)threadIdx.x >> 6);
+    const int c = u / np.nrows, row = u - c * np.nrows;
+    if (c < np.chunks) np_field_one(d, Acc{d.F, d.nbp}, row, c * 64 + ((int)threadIdx.x & 63));
     return true;
 }
+
+// A round's partner block, loaded by the whole wavefront one round ahead:
+// slot s = lane + 64 m holds field s / L of partner q0 + s % L (= lane % L),
+// so the loads of one field are L consecutive doubles, and the global-memory
+// latency of the next round overlaps this round's arithmetic.  In LDS,
+// partner u's fields start at u * P (P odd: the partners' rows start in
+// different banks).
+template <int L, int NF>
+struct PartnerBlock {
+    static constexpr int NS = (L * NF + 63) / 64;
+    static constexpr int P = NF | 1;
+    int64_t fo[NS];
+    int dst[NS];
+    bool has[NS];
+    double pf[NS];
+    __device__ __forceinline__ void init(const int *fields, int64_t nbp, int lane) {
+#pragma unroll
+        for (int m = 0; m < NS; ++m) {
+            const int sl = lane + 64 * m, f = sl / L;
+            has[m] = sl < L * NF;
+            fo[m] = has[m] ? (int64_t)fields[f] * nbp : 0;
+            dst[m] = (sl % L) * P + f;
+        }
+    }
+    __device__ __forceinline__ void fetch(const double *__restrict__ F, int off, int n, int q0, int k) {
+        const int jl = q0 + k;
+#pragma unroll
+        for (int m = 0; m < NS; ++m) pf[m] = (has[m] && jl < n) ? F[fo[m] + off + jl] : 0.0;
+    }
+    __device__ __forceinline__ void store(double *jb) const {
+#pragma unroll
+        for (int m = 0; m < NS; ++m)
+            if (has[m]) jb[dst[m]] = pf[m];
+    }
+};
 
 // ---------------------------------------------------------------------------
 // deform + spin fixed point + PREPARE_RELATIVITY (mass.cpp:78-135)
@@ -429,34 +478,51 @@ __global__ __launch_bounds__(BS) void k_deform_light(View d, const int4 *__restr
 // subtracted terms stored negated.  The self pair and partners past the end
 // add +0, an exact identity.
 constexpr int kDefK4Pitch = 13;
+// fields of a deform partner: r (hi, lo), sR2, GM; then v (hi, lo) and the
+// tide delay factor for constant-time-lag bodies
+constexpr int kJDefF = 8, kJDefDampedF = 15;
+__constant__ int c_jdef_field[kJDefDampedF] = {F_R + 0, F_R + 1, F_R + 2, F_R + 3, F_R + 4, F_R + 5, F_SR2, F_GM,
+                                               F_V + 0, F_V + 1, F_V + 2, F_V + 3, F_V + 4, F_V + 5, F_TDF};
+
 __global__ __launch_bounds__(64) void k_deform_k4(View d, const int4 *__restrict__ blk, int prepare, NpTail np) {
     if (np_tail(d, np)) return;
     const Acc A{d.F, d.nbp};
     const int4 bk = blk[blockIdx.x];
+    if (bk.z <= 0) return;                                 // padding of an XCD-dealt list
     const SysMeta sm = d.sys[bk.x];
     const int n = sm.n, off = sm.off;
     const int lane = (int)threadIdx.x, b = lane >> 2, k = lane & 3;
     const bool valid = b < bk.z;
     const int gi = valid ? d.perm[off + bk.y + b] : off;
     const int il = gi - off;
+    using Blk = PartnerBlock<4, kJDefF>;
     __shared__ double term[64 * kDefK4Pitch];
+    __shared__ double jb[4 * Blk::P];
+    Blk pb;
+    pb.init(c_jdef_field, d.nbp, lane);
+    pb.fetch(d.F, off, n, 0, k);
+    pb.store(jb);
     const ddv3 ri = A.lddv(F_R, gi);
     const double sR2 = A.ld(F_SR2, gi), k2 = A.ld(F_K2, gi);
     const bool ndelay = valid && k2 != 0;
     const bool any_nd = __ballot(ndelay) != 0;             // wave-uniform
     double acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
     const double *grp = term + (b * 4) * kDefK4Pitch;
+    __syncthreads();
     for (int q0 = 0; q0 < n; q0 += 4) {
         const int jl = q0 + k;
+        const bool more = q0 + 4 < n;
+        if (more) pb.fetch(d.F, off, n, q0 + 4, k);
         double t[13];
 #pragma unroll
         for (int r = 0; r < 13; ++r) t[r] = 0.0;
         if (valid && jl < n && jl != il) {
-            const int gj = off + jl;
-            const v3 r = diff(A.lddv(F_R, gj), ri);
-            const double rr2 = 1 / smax(dot(r, r), sR2 + A.ld(F_SR2, gj));
+            const double *J = jb + k * Blk::P;
+            const ddv3 rj{dd{J[0], J[1]}, dd{J[2], J[3]}, dd{J[4], J[5]}};
+            const v3 r = diff(rj, ri);
+            const double rr2 = 1 / smax(dot(r, r), sR2 + J[6]);
             const double rr = sqrt(rr2);
-            const double tp_dphi = rr * A.ld(F_GM, gj);
+            const double tp_dphi = rr * J[7];
             const double tp_dg = rr2 * tp_dphi;
             const v3 na = tp_dg * r;
             t[0] = -tp_dphi;                               // _NK2 (mass.impl:38-46) / _NDELAY (:25-36)
@@ -487,6 +553,7 @@ __global__ __launch_bounds__(64) void k_deform_k4(View d, const int4 *__restrict
                 if (k == 0) acc3 += tr[12];
             }
         }
+        if (more) pb.store(jb);                            // every lane has read this round's block
         __syncthreads();
     }
     // gather the body's chains into its lane 0
@@ -765,6 +832,7 @@ constexpr int kK4Pitch = 25;
 __global__ __launch_bounds__(64) void k_main_k4(View d, const int2 *__restrict__ blk) {
     const Acc A{d.F, d.nbp};
     const int s = blk[blockIdx.x].x, i0 = blk[blockIdx.x].y;   // bodies i0 .. i0 + 15
+    if (i0 < 0) return;                                    // padding of an XCD-dealt list
     const SysMeta sm = d.sys[s];
     const int n = sm.n, off = sm.off;
     const int lane = (int)threadIdx.x, b = lane >> 2, k = lane & 3;
@@ -1064,11 +1132,15 @@ __global__ __launch_bounds__(64 * W) void k_main_wpb(View d, const int2 *__restr
 constexpr int kWpbDefPitch = 65;
 // W waves per body: in each round wave w evaluates partner chunk
 // (round * W + w); wave 0 then adds the W chunks in partner order.
-template <int W>
+// TAIL: the launch carries the non-point item fields in extra workgroups
+// (without them the kernel needs no scratch).  Work items with il < 0 are
+// padding of an XCD-dealt list.
+template <int W, bool TAIL>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 1 ? 3 : 1))) void k_deform_wpb(View d, const int2 *__restrict__ work, int prepare, NpTail np) {
-    if (np_tail(d, np)) return;
+    if (TAIL && np_tail(d, np)) return;
     const Acc A{d.F, d.nbp};
     const int s = work[blockIdx.x].x, il = work[blockIdx.x].y;
+    if (il < 0) return;
     const SysMeta sm = d.sys[s];
     const int n = sm.n, off = sm.off, gi = off + il;
     const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
@@ -1212,8 +1284,8 @@ __device__ __forceinline__ void serial_reaction(const View &d, const Acc &A, int
     const v3 dg = (p.rr3 * p.rr2 * T.R2) * quad_field(Cp, p);
     da -= T.GM * dg;
     const int gpi = d.np_gp[gp_], rgi = d.np_ring[gp_];
-    if (gpi >= 0) da += T.GM * nonpoint_field(d, A, gp_, 0, gpi, p.r);
-    if (rgi >= 0) da += T.GM * nonpoint_field(d, A, gp_, 1, rgi, p.r);
+    if (gpi >= 0) da += T.GM * nonpoint_field_call(d, A, gp_, 0, gpi, p.r);
+    if (rgi >= 0) da += T.GM * nonpoint_field_call(d, A, gp_, 1, rgi, p.r);
 }
 
 template <int BS>
@@ -1252,12 +1324,12 @@ __global__ __launch_bounds__(BS) void k_main_serial(View d, const int2 *__restri
                     dq += T.GM * cross(p.r, dg);
                     lt_rad(I, T, p, da, dq);
                     if (my_gp >= 0) {                      // APPLY_NONPOINT_FORCE, mass.impl:127-130
-                        const v3 an = nonpoint_field(d, A, gi, 0, my_gp, p.r);
+                        const v3 an = nonpoint_field_call(d, A, gi, 0, my_gp, p.r);
                         da -= T.GM * an;
                         dq -= T.GM * cross(p.r, an);
                     }
                     if (my_ring >= 0) {
-                        const v3 an = nonpoint_field(d, A, gi, 1, my_ring, p.r);
+                        const v3 an = nonpoint_field_call(d, A, gi, 1, my_ring, p.r);
                         da -= T.GM * an;
                         dq -= T.GM * cross(p.r, an);
                     }
@@ -1278,7 +1350,7 @@ __global__ __launch_bounds__(BS) void k_main_serial(View d, const int2 *__restri
 // ---------------------------------------------------------------------------
 // non-point items, thread-pool order (mass.cpp:158-214, 253-307)
 // ---------------------------------------------------------------------------
-__global__ void k_np_field(View d, int nrows) {
+__global__ __launch_bounds__(64) void k_np_field(View d, int nrows) {
     // one thread per (item, partner): an = field of the item at r_j - r_p
     const Acc A{d.F, d.nbp};
     const int row = blockIdx.y;
@@ -1286,12 +1358,15 @@ __global__ void k_np_field(View d, int nrows) {
     np_field_one(d, A, row, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-__global__ void k_np_item(View d, int nrows, int maxT) {
-    // one thread per (item, round k): the item's partial sums over the slice
-    // it visits in round k (tpmi of accel_nonpoint)
+__global__ __launch_bounds__(64) void k_np_item(View d, int nrows, int maxT) {
+    // one wavefront per (item, round k): the item's partial sums over the
+    // slice it visits in round k (tpmi of accel_nonpoint).  Lanes form the
+    // partners' terms (subtracted terms stored negated, so a - b becomes
+    // a + (-b), the same value); lane c < 6 adds chain c in partner order
+    // (td x, y, z, tq x, y, z).  The self pair and lanes past the slice add
+    // +0, an exact identity for sums that start at +0.
     const Acc A{d.F, d.nbp};
-    const int row = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = row % maxT, it_i = row / maxT;
+    const int k = (int)blockIdx.x % maxT, it_i = (int)blockIdx.x / maxT;
     if (it_i >= nrows) return;
     const NPItem it = d.items[it_i];
     const SysMeta sm = d.sys[it.sys];
@@ -1300,32 +1375,58 @@ __global__ void k_np_item(View d, int nrows, int maxT) {
     int kk = it.chain + k;
     if (kk >= T) kk -= T;
     const int j0 = kk * sm.n / T, j1 = (kk + 1) * sm.n / T;
+    const int lane = (int)threadIdx.x;
     const ddv3 rp = A.lddv(F_R, it.body);
-    v3 td{0, 0, 0}, tq{0, 0, 0};
-    for (int jl = j0; jl < j1; ++jl) {
-        const int gj = sm.off + jl;
-        if (gj == it.body) continue;
-        const v3 r = diff(A.lddv(F_R, gj), rp);
-        const double *a = d.an + 3 * ((int64_t)it.an_off + jl);
-        const v3 an{a[0], a[1], a[2]};
-        const double GMj = A.ld(F_GM, gj);
-        td -= GMj * an;
-        tq -= GMj * cross(r, an);
+    __shared__ double term[64 * 7];
+    double acc = 0;
+    for (int base = j0; base < j1; base += 64) {
+        const int jl = base + lane;
+        double t[6] = {0, 0, 0, 0, 0, 0};
+        if (jl < j1 && sm.off + jl != it.body) {
+            const int gj = sm.off + jl;
+            const v3 r = diff(A.lddv(F_R, gj), rp);
+            const double *a = d.an + 3 * ((int64_t)it.an_off + jl);
+            const v3 an{a[0], a[1], a[2]};
+            const double GMj = A.ld(F_GM, gj);
+            const v3 ta = GMj * an, tq = GMj * cross(r, an);
+            t[0] = -ta.x; t[1] = -ta.y; t[2] = -ta.z;
+            t[3] = -tq.x; t[4] = -tq.y; t[5] = -tq.z;
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) term[lane * 7 + c] = t[c];
+        __syncthreads();
+        if (lane < 6) {
+            const int cnt = min(64, j1 - base);
+            for (int q = 0; q < cnt; ++q) acc += term[q * 7 + lane];
+        }
+        __syncthreads();
     }
-    double *o = d.part + 6 * ((int64_t)it_i * maxT + k);
-    o[0] = td.x; o[1] = td.y; o[2] = td.z; o[3] = tq.x; o[4] = tq.y; o[5] = tq.z;
+    if (lane < 6) d.part[6 * ((int64_t)it_i * maxT + k) + lane] = acc;
 }
 
-__global__ void k_np_recv(View d, const int2 *__restrict__ blk, int maxT) {
+constexpr int kRecvStaged = 64;          // items per system staged in LDS
+__global__ __launch_bounds__(kBS) void k_np_recv(View d, const int2 *__restrict__ blk, int maxT) {
     // one thread per body of a system with items: receiver terms in round
-    // order, then the body's own item sums, finalisation and ring correction
+    // order, then the body's own item sums, finalisation and ring correction.
+    // The system's items (body, row offset, GM of the body) are staged in LDS
+    // once per workgroup.
     const Acc A{d.F, d.nbp};
     const int s = blk[blockIdx.x].x, i0 = blk[blockIdx.x].y;
     const SysMeta sm = d.sys[s];
-    const int jl = i0 + (int)threadIdx.x;
-    if (jl >= sm.n) return;
-    const int gj = sm.off + jl;
     const int T = sm.T, nit = sm.nitem, n = sm.n;
+    __shared__ int s_body[kRecvStaged], s_an[kRecvStaged];
+    __shared__ double s_gm[kRecvStaged];
+    const bool staged = nit <= kRecvStaged;
+    if (staged && (int)threadIdx.x < nit) {
+        const NPItem it = d.items[sm.item0 + threadIdx.x];
+        s_body[threadIdx.x] = it.body;
+        s_an[threadIdx.x] = it.an_off;
+        s_gm[threadIdx.x] = A.ld(F_GM, it.body);
+    }
+    __syncthreads();
+    const int jl = i0 + (int)threadIdx.x;
+    if (jl >= n) return;
+    const int gj = sm.off + jl;
     v3 da = A.ldv(F_DACC, gj), dq = A.ldv(F_DTQ, gj);
     int sj = 0;
     while ((sj + 1) * n / T <= jl) ++sj;
@@ -1334,10 +1435,17 @@ __global__ void k_np_recv(View d, const int2 *__restrict__ blk, int maxT) {
         if (c < 0) c += T;
         const int first = c * nit / T, last = (c + 1) * nit / T;
         for (int q = last - 1; q >= first; --q) {
-            const NPItem it = d.items[sm.item0 + q];
-            if (it.body == gj) continue;
-            const double *a = d.an + 3 * ((int64_t)it.an_off + jl);
-            da += A.ld(F_GM, it.body) * v3{a[0], a[1], a[2]};
+            int body, an_off;
+            double gm;
+            if (staged) {
+                body = s_body[q]; an_off = s_an[q]; gm = s_gm[q];
+            } else {
+                const NPItem it = d.items[sm.item0 + q];
+                body = it.body; an_off = it.an_off; gm = A.ld(F_GM, it.body);
+            }
+            if (body == gj) continue;
+            const double *a = d.an + 3 * ((int64_t)an_off + jl);
+            da += gm * v3{a[0], a[1], a[2]};
         }
     }
     // own items: idaccel (geopotential) then jdaccel (ring), mass.cpp:298-307
@@ -1363,7 +1471,7 @@ __global__ void k_np_recv(View d, const int2 *__restrict__ blk, int maxT) {
 // ---------------------------------------------------------------------------
 // RK12 stage: store f[k-1], combine, update state (RungeKutta.impl:113-186)
 // ---------------------------------------------------------------------------
-__device__ void stage_body(const View &d, const Acc &A, int g, int k, double dt, double dt_k, double tadd) {
+__device__ __forceinline__ void stage_body_inl(const View &d, const Acc &A, int g, int k, double dt, double dt_k, double tadd) {
     ddv3 v = A.lddv(F_V, g);
     if (k == 1) {
         A.stddv(F_X0R, g, A.lddv(F_R, g));
@@ -1432,16 +1540,21 @@ __device__ void stage_body(const View &d, const Acc &A, int g, int k, double dt,
     A.st(F_EXJ2, g, A.ld(F_DJ2, g) * t);
 }
 
+// the persistent small-system kernel keeps the stage update out of line
+__device__ __noinline__ void stage_body(const View &d, const Acc &A, int g, int k, double dt, double dt_k, double tadd) {
+    stage_body_inl(d, A, g, k, dt, dt_k, tadd);
+}
+
 // istep != nullptr: the step index comes from device memory (replayed graphs)
 // and the stage time offset dt * (dt_k + i_step) is formed here, with the
 // same two operations as on the host.
-__global__ __launch_bounds__(64) void k_stage(View d, int k, double dt, double dt_k, double tadd,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_stage(View d, int k, double dt, double dt_k, double tadd,
                                               const int64_t *__restrict__ istep) {
     const Acc A{d.F, d.nbp};
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= d.nb) return;
     if (istep) tadd = dt * (dt_k + (double)*istep);
-    stage_body(d, A, g, k, dt, dt_k, tadd);
+    stage_body_inl(d, A, g, k, dt, dt_k, tadd);
 }
 
 __global__ void k_step_count(int64_t *istep) {
@@ -1542,12 +1655,12 @@ __device__ void main_serial_body_tile(const View &d, const Acc &A, int gi, int i
         dq += T.GM * cross(p.r, dg);
         lt_rad(I, T, p, da, dq);
         if (my_gp >= 0) {
-            const v3 an = nonpoint_field(d, A, gi, 0, my_gp, p.r);
+            const v3 an = nonpoint_field_call(d, A, gi, 0, my_gp, p.r);
             da -= T.GM * an;
             dq -= T.GM * cross(p.r, an);
         }
         if (my_ring >= 0) {
-            const v3 an = nonpoint_field(d, A, gi, 1, my_ring, p.r);
+            const v3 an = nonpoint_field_call(d, A, gi, 1, my_ring, p.r);
             da -= T.GM * an;
             dq -= T.GM * cross(p.r, an);
         }
@@ -1782,24 +1895,30 @@ void launch_main_wpb(const View &d, const int2 *work, int nwork, int max_n, bool
     else if (max_n > 64 && nwork <= 1536) hipLaunchKernelGGL((k_main_wpb<false, 2>), dim3(nwork), dim3(128), 0, st, d, work);
     else hipLaunchKernelGGL((k_main_wpb<false, 1>), dim3(nwork), dim3(64), 0, st, d, work);
 }
-// Wavefronts per body: as many as the partner chunks need while every body's
-// workgroup still fits on the GPU at once (the ordered sums are serial, so
-// extra wavefronts only pay while workgroups would otherwise be idle
-// CUs; a large batch is faster with one wavefront per body).
+// Wavefronts per body: several while every body's workgroup still fits on
+// the GPU at once (the ordered sums are serial, so extra wavefronts only pay
+// while CUs would otherwise be idle; a large batch is faster with one
+// wavefront per body).  Four, not eight, for a 504-body system: two 256-lane
+// workgroups then share each CU, one round of workgroups instead of two
+// (31 us instead of 43-58 us per launch on one MI355X).
 void launch_deform_wpb(const View &d, const int2 *work, int nwork, int max_n, int prepare, int np_rows,
                        hipStream_t st) {
     if (!nwork && !np_rows) return;
     const int chunks = (max_n + 63) / 64;
     int W = 1;
-    if (chunks >= 8 && nwork <= 512) W = 8;
-    else if (chunks >= 4 && nwork <= 1280) W = 4;
+    if (chunks >= 4 && nwork <= 1280) W = 4;
     else if (chunks >= 2 && nwork <= 2816) W = 2;
-    const NpTail np{nwork, np_rows, (max_n + 64 * W - 1) / (64 * W)};
-    const dim3 grid(nwork + np_rows * np.chunks), block(64 * W);
-    if (W == 8) hipLaunchKernelGGL(k_deform_wpb<8>, grid, block, 0, st, d, work, prepare, np);
-    else if (W == 4) hipLaunchKernelGGL(k_deform_wpb<4>, grid, block, 0, st, d, work, prepare, np);
-    else if (W == 2) hipLaunchKernelGGL(k_deform_wpb<2>, grid, block, 0, st, d, work, prepare, np);
-    else hipLaunchKernelGGL(k_deform_wpb<1>, grid, block, 0, st, d, work, prepare, np);
+    const NpTail np{nwork, np_rows, (max_n + 63) / 64};
+    const dim3 grid(nwork + (np_rows * np.chunks + W - 1) / W), block(64 * W);
+    if (np_rows) {
+        if (W == 4) hipLaunchKernelGGL((k_deform_wpb<4, true>), grid, block, 0, st, d, work, prepare, np);
+        else if (W == 2) hipLaunchKernelGGL((k_deform_wpb<2, true>), grid, block, 0, st, d, work, prepare, np);
+        else hipLaunchKernelGGL((k_deform_wpb<1, true>), grid, block, 0, st, d, work, prepare, np);
+    } else {
+        if (W == 4) hipLaunchKernelGGL((k_deform_wpb<4, false>), grid, block, 0, st, d, work, prepare, np);
+        else if (W == 2) hipLaunchKernelGGL((k_deform_wpb<2, false>), grid, block, 0, st, d, work, prepare, np);
+        else hipLaunchKernelGGL((k_deform_wpb<1, false>), grid, block, 0, st, d, work, prepare, np);
+    }
 }
 void launch_main_serial(const View &d, const int2 *blk, int nblk, hipStream_t st) {
     if (nblk) hipLaunchKernelGGL(k_main_serial<kBS>, dim3(nblk), dim3(kBS), 0, st, d, blk);
@@ -1811,8 +1930,7 @@ void launch_np(const View &d, int nrows, int max_n, int maxT, const int2 *recv_b
                hipStream_t st) {
     if (!nrows) return;
     if (with_field) hipLaunchKernelGGL(k_np_field, dim3((max_n + 63) / 64, nrows), dim3(64), 0, st, d, nrows);
-    const int nthr = nrows * maxT;
-    hipLaunchKernelGGL(k_np_item, dim3((nthr + 63) / 64), dim3(64), 0, st, d, nrows, maxT);
+    hipLaunchKernelGGL(k_np_item, dim3(nrows * maxT), dim3(64), 0, st, d, nrows, maxT);
     if (nrecv) hipLaunchKernelGGL(k_np_recv, dim3(nrecv), dim3(kBS), 0, st, d, recv_blk, maxT);
 }
 void launch_stage(const View &d, int k, double dt, double dt_k, double tadd, const int64_t *istep, hipStream_t st) {

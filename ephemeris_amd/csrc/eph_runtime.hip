@@ -145,6 +145,51 @@ static void append_blocks(std::vector<int2> &blk, int s, int n) {
     for (int i0 = 0; i0 < n; i0 += kBS) blk.push_back(make_int2(s, i0));
 }
 
+// Workgroup b of a launch is dispatched to XCD b % 8, and each XCD has its
+// own L2.  A work list of many systems is dealt so that every workgroup of
+// one system runs on one XCD: that L2 then holds the system's partner data
+// and no other XCD fetches it.  Systems go, largest first, to the XCD with
+// the least work so far; position 8 r + x of the result holds the r-th item
+// of XCD x, and shorter shares are padded with `empty` (which the kernels
+// skip).  Lists of few systems, or with one system of more than an eighth of
+// the work, are left as they are: spreading each system over every XCD then
+// balances better.  Only the placement changes, never any result.
+static constexpr int kXcds = 8;
+
+template <typename T, typename SysOf, typename Cost>
+static std::vector<T> xcd_deal(const std::vector<T> &items, int64_t n_sys, SysOf sys_of, Cost cost, T empty) {
+    std::vector<std::vector<int>> by_sys(n_sys);
+    std::vector<double> w(n_sys, 0.0);
+    double total = 0, wmax = 0;
+    for (int i = 0; i < (int)items.size(); ++i) {
+        const int s = sys_of(items[i]);
+        by_sys[s].push_back(i);
+        w[s] += cost(items[i]);
+    }
+    std::vector<int> order;
+    for (int64_t s = 0; s < n_sys; ++s)
+        if (!by_sys[s].empty()) {
+            order.push_back((int)s);
+            total += w[s];
+            wmax = std::max(wmax, w[s]);
+        }
+    if ((int)order.size() < 2 * kXcds || wmax * kXcds > total) return items;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return w[a] > w[b]; });
+    std::array<std::vector<T>, kXcds> share;
+    std::array<double, kXcds> load{};
+    for (int s : order) {
+        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[x] += w[s];
+        for (int i : by_sys[s]) share[x].push_back(items[i]);
+    }
+    size_t len = 0;
+    for (const auto &sh : share) len = std::max(len, sh.size());
+    std::vector<T> out(len * kXcds, empty);
+    for (int x = 0; x < kXcds; ++x)
+        for (size_t r = 0; r < share[x].size(); ++r) out[r * kXcds + x] = share[x][r];
+    return out;
+}
+
 int Batch::build_plan(int mode) {
     Plan &pl = plan[mode];
     std::vector<SysMeta> meta(n_sys);
@@ -208,6 +253,8 @@ int Batch::build_plan(int mode) {
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sys_n[a] > sys_n[b]; });
         for (int s : order)
             for (int i0 = 0; i0 < (int)sys_n[s]; i0 += 16) blk16.push_back(make_int2(s, i0));
+        blk16 = xcd_deal(blk16, n_sys, [](int2 b) { return b.x; },
+                         [&](int2 b) { return (double)sys_n[b.x]; }, make_int2(0, -1));
     }
     int rc;
     if ((rc = dev_upload(*this, pl.d_blk16, blk16))) return rc;
@@ -282,7 +329,8 @@ int Batch::accel(int mode, bool prepare_only) {
             launch_deform(v, d_blk_light, n_blk_light, d_blk_damped, n_blk_damped, prepare_only ? 0 : 1, stream);
         } else {
             // constant-time-lag bodies re-sum their partners once per spin pass:
-            // one wavefront each; the rest four lanes each
+            // one wavefront each; the rest four lanes each (that launch
+            // carries the item fields)
             launch_deform_wpb(v, d_work_damped, n_work_damped, max_sys_n, prepare_only ? 0 : 1, 0, stream);
             launch_deform_k4(v, d_blk_light16, n_blk_light16, prepare_only ? 0 : 1, np_rows, pl.max_n, stream);
             field_done = true;
@@ -527,6 +575,11 @@ int Batch::build_class_tables(const eph_mass *masses) {
         for (int64_t q = nd; q < sys_n[s]; q += 16)
             light16.push_back(make_int4((int)s, (int)q, (int)std::min<int64_t>(16, sys_n[s] - q), 0));
     }
+    // constant-time-lag bodies re-sum their partners once per spin pass
+    damped_work = xcd_deal(damped_work, n_sys, [](int2 b) { return b.x; },
+                           [&](int2 b) { return (double)sys_n[b.x]; }, make_int2(0, -1));
+    light16 = xcd_deal(light16, n_sys, [](int4 b) { return b.x; },
+                       [&](int4 b) { return (double)b.z * sys_n[b.x]; }, make_int4(0, 0, 0, 0));
     HIP_CHECK(hipStreamSynchronize(stream));
     for (void *p : class_allocs) (void)hipFree(p);
     class_allocs.clear();

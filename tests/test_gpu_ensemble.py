@@ -68,3 +68,23 @@ def test_autotuned_batch_matches_fixed_variant(tmp_path_factory):
     b, tb = fixed.download()
     bad = [f for f in a.dtype.names if not bits_equal(a[f], b[f])]
     assert not bad and bits_equal(ta, tb), bad
+
+
+def test_xcd_dealt_batch_matches_per_thread_variant(tmp_path_factory):
+    """A batch of 24 systems on the four-lanes-per-body kernels, whose work
+    lists are dealt to the XCDs by system and padded with empty items, gives
+    the bits of the one-thread-per-body kernels."""
+    print("\n[case] 24-member batch, variant 4 (XCD-dealt lists) vs variant 1, 2 steps", flush=True)
+    specs = gen.make_ensemble(24, seed=97, n_lo=64, n_hi=200)
+    cfgs = ensemble.write_members(specs, tmp_path_factory.mktemp("xcd"))
+    masses, sys_n, gps, rings, t0 = ensemble.load_members(cfgs)
+    k4 = GpuSystems(masses, sys_n, gps, rings, t0)
+    pt = GpuSystems(masses, sys_n, gps, rings, t0)
+    k4.set_variant(4)
+    pt.set_variant(1)
+    k4.integrate(300.0, 2)
+    pt.integrate(300.0, 2)
+    a, ta = k4.download()
+    b, tb = pt.download()
+    bad = [f for f in a.dtype.names if not bits_equal(a[f], b[f])]
+    assert not bad and bits_equal(ta, tb), bad
