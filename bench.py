@@ -247,11 +247,11 @@ def main():
     main_ms = prof["main_force_pool"][0]
     flop_main = n * (n - 1) * FLOP_MAIN_PAIR
     achieved = flop_main / (main_ms * 1e-3) / 1e12
-    roofline = {"kernel": "k_main_wpb<true, 2> (main force + fused non-point pass)", "kernel_class": "main_force_pool",
+    roofline = {"kernel": "k_main_wpb<true, 4> (main force + fused non-point pass)", "kernel_class": "main_force_pool",
                 "dominant_kernel": dominant, "bound": "mfma", "compute": "FP64 vector ALU (same 78.6 TFLOP/s roof; no MFMA used)",
                 "achieved": achieved, "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP64_VECTOR_PEAK_TFLOPS,
-                "traffic": profiled_traffic("void eph::k_main_wpb<true, 2>", n * 128), "traffic_unit": "bytes per launch",
+                "traffic": profiled_traffic("void eph::k_main_wpb<true, 4>", n * 256), "traffic_unit": "bytes per launch",
                 "flop_per_launch": flop_main, "avg_launch_ms": main_ms,
                 "kernel_ms": {k: round(v[0], 5) for k, v in prof.items()},
                 "kernel_launches": {k: v[1] for k, v in prof.items()}}

@@ -16,6 +16,10 @@ namespace eph {
 // tables
 // ---------------------------------------------------------------------------
 __constant__ double2 c_rk[325];                    // Feagin RK12, {hi, lo}
+// Non-zero coefficients of each row k = 1..25: entries c_row0[k] ..
+// c_row0[k + 1] - 1 of c_nzj hold their column j, ascending (set with c_rk)
+__constant__ int c_nzj[325];
+__constant__ int c_row0[27];
 
 __device__ const float g_fz[32 * 64 * 16] = {
 #include "../tables/ring_fzcorr.inc"
@@ -996,6 +1000,7 @@ __device__ __forceinline__ void wpb_nonpoint(const View &d, const Acc &A, const 
         const int q = own_q[kind];
         if (q < 0) continue;
         const NPItem it = d.items[sm.item0 + q];
+#pragma unroll 4
         for (int jl = lane; jl < n; jl += nthreads) {     // pq[j]: GM_j * field at j, GM_j * r x field
             v3 ta{0, 0, 0}, tt{0, 0, 0};
             if (jl != il) {
@@ -1015,6 +1020,7 @@ __device__ __forceinline__ void wpb_nonpoint(const View &d, const Acc &A, const 
             if (kk >= T) kk -= T;
             const int j0 = kk * n / T, j1 = (kk + 1) * n / T;
             v3 td{0, 0, 0}, tq{0, 0, 0};
+#pragma unroll 8
             for (int jl = j0; jl < j1; ++jl) {             // the self pair holds +0: an exact identity
                 const double *o = pq + 6 * jl;
                 td -= v3{o[0], o[1], o[2]};
@@ -1130,6 +1136,28 @@ __global__ __launch_bounds__(64 * W) void k_main_wpb(View d, const int2 *__restr
 // subtracted terms (phi, tidal matrix) are stored negated and the self pair
 // contributes +0, as in k_main_wpb.
 constexpr int kWpbDefPitch = 65;
+
+// acc += row[0] + ... + row[63], one dependent addition at a time in index
+// order; the reads of the next eight values are issued before this eight
+// are added, so the chain does not wait for LDS.
+__device__ __forceinline__ void chain_add64(const double *row, double &acc) {
+    double b0[8], b1[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) b0[i] = row[i];
+#pragma unroll
+    for (int q = 0; q < 64; q += 16) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) b1[i] = row[q + 8 + i];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) acc += b0[i];
+        if (q + 16 < 64) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) b0[i] = row[q + 16 + i];
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) acc += b1[i];
+    }
+}
 // W waves per body: in each round wave w evaluates partner chunk
 // (round * W + w); wave 0 then adds the W chunks in partner order.
 // TAIL: the launch carries the non-point item fields in extra workgroups
@@ -1224,11 +1252,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 1 ?
             __syncthreads();
             if (wv == 0 && lane < nacc_chains && (first_pass || lane >= 4)) {
                 const int nw = min(W, (n - t0 + 63) / 64);
-                for (int u = 0; u < nw; ++u) {
-                    const double *row = term[u] + lane * kWpbDefPitch;
-#pragma unroll 16
-                    for (int q = 0; q < 64; ++q) acc += row[q];
-                }
+                for (int u = 0; u < nw; ++u) chain_add64(term[u] + lane * kWpbDefPitch, acc);
             }
             __syncthreads();
         }
@@ -1828,18 +1852,21 @@ __global__ __launch_bounds__(kPartBS) void k_particles(PView pv, int n_steps, do
             ddv3 cv = mkddv(v3{0, 0, 0});
             v3 ca{0, 0, 0};
             const int base = k * (k - 1) / 2;
-            for (int j = 0; j < k; ++j) {
+            const int e1 = c_row0[k + 1];
+            for (int e = c_row0[k]; e < e1; ++e) {             // non-zero columns, ascending
+                const int j = c_nzj[e];
                 const double2 c = c_rk[base + j];
-                if (c.x == 0) continue;
                 const dd ckj{c.x, c.y};
                 const double fc = c.x + c.y;
                 ddv3 vj = v0;
                 if (j > 0) {
                     v3 caj{0, 0, 0};
                     const int bj = j * (j - 1) / 2;
-                    for (int l = 0; l < j; ++l) {
+                    const int f1 = c_row0[j + 1];
+#pragma unroll 4
+                    for (int f = c_row0[j]; f < f1; ++f) {
+                        const int l = c_nzj[f];
                         const double2 cl = c_rk[bj + l];
-                        if (cl.x == 0) continue;
                         caj += (cl.x + cl.y) * v3{FA(l, 0), FA(l, 1), FA(l, 2)};
                     }
                     vj = add(v0, mkddv(dt * caj));
@@ -1871,6 +1898,17 @@ void launch_particles(const PView &pv, int n_steps, double dt, const DtK &dtk, h
 // ---------------------------------------------------------------------------
 void set_rk_table(const double *tab650, hipStream_t st) {
     (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(c_rk), tab650, sizeof(double) * 650, 0, hipMemcpyHostToDevice, st);
+    static int nzj[325], row0[27];
+    int e = 0;
+    for (int k = 0; k <= 25; ++k) {
+        row0[k] = e;
+        if (k == 0) continue;
+        for (int j = 0; j < k; ++j)
+            if (tab650[2 * (k * (k - 1) / 2 + j)] != 0) nzj[e++] = j;
+    }
+    row0[26] = e;
+    (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(c_nzj), nzj, sizeof(nzj), 0, hipMemcpyHostToDevice, st);
+    (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(c_row0), row0, sizeof(row0), 0, hipMemcpyHostToDevice, st);
 }
 
 void launch_deform_k4(const View &d, const int4 *blk16, int nblk, int prepare, int np_rows, int max_n, hipStream_t st) {
@@ -1889,9 +1927,16 @@ void launch_main_pool(const View &d, const int2 *blk, int nblk, hipStream_t st) 
 void launch_main_k4(const View &d, const int2 *blk16, int nblk, hipStream_t st) {
     if (nblk) hipLaunchKernelGGL(k_main_k4, dim3(nblk), dim3(64), 0, st, d, blk16);
 }
+// Four wavefronts per body while every body's workgroup is resident at once
+// (52 KB of LDS each: three per CU): the pair terms of a 504-body system are
+// formed in two rounds instead of four (38 us instead of 44 us per launch on
+// one MI355X); the ordered sums are the same.
 void launch_main_wpb(const View &d, const int2 *work, int nwork, int max_n, bool fused_np, hipStream_t st) {
     if (!nwork) return;
-    if (fused_np) hipLaunchKernelGGL((k_main_wpb<true, 2>), dim3(nwork), dim3(128), 0, st, d, work);
+    const bool four = max_n > 128 && nwork <= 768;
+    if (fused_np && four) hipLaunchKernelGGL((k_main_wpb<true, 4>), dim3(nwork), dim3(256), 0, st, d, work);
+    else if (fused_np) hipLaunchKernelGGL((k_main_wpb<true, 2>), dim3(nwork), dim3(128), 0, st, d, work);
+    else if (four) hipLaunchKernelGGL((k_main_wpb<false, 4>), dim3(nwork), dim3(256), 0, st, d, work);
     else if (max_n > 64 && nwork <= 1536) hipLaunchKernelGGL((k_main_wpb<false, 2>), dim3(nwork), dim3(128), 0, st, d, work);
     else hipLaunchKernelGGL((k_main_wpb<false, 1>), dim3(nwork), dim3(64), 0, st, d, work);
 }
