@@ -1495,11 +1495,43 @@ __global__ __launch_bounds__(kBS) void k_np_recv(View d, const int2 *__restrict_
 // ---------------------------------------------------------------------------
 // RK12 stage: store f[k-1], combine, update state (RungeKutta.impl:113-186)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void stage_body_inl(const View &d, const Acc &A, int g, int k, double dt, double dt_k, double tadd) {
-    ddv3 v = A.lddv(F_V, g);
+// The stage update splits into two independent halves: the translational
+// one (r, v, the stage's acceleration, GM(t), exJ2(t)) and the rotational one
+// (GL, the frame s, w, torque, Egrad).  Neither reads a field the other
+// writes, so k_stage runs them in different workgroups.
+__device__ __forceinline__ void stage_translational(const View &d, const Acc &A, int g, int k, double dt, double tadd) {
+    const ddv3 v = A.lddv(F_V, g);
     if (k == 1) {
         A.stddv(F_X0R, g, A.lddv(F_R, g));
         A.stddv(F_X0V, g, v);
+    }
+    {
+        const int fb = F_STAGE + (k - 1) * kStageFields;
+        A.stddv(fb, g, v);
+        A.stv(fb + 6, g, (A.ldv(F_GACC, g) + A.ldv(F_DACC, g)) + A.ldv(F_NACC, g));
+    }
+    ddv3 cv = mkddv(v3{0, 0, 0});
+    v3 ca{0, 0, 0};
+    const int base = k * (k - 1) / 2;
+    for (int j = 0; j < k; ++j) {
+        const double2 c = c_rk[base + j];
+        if (c.x == 0) continue;
+        const dd ckj{c.x, c.y};
+        const double fc = c.x + c.y;
+        const int fb = F_STAGE + j * kStageFields;
+        cv = add(cv, scale(ckj, A.lddv(fb, g)));
+        ca += fc * A.ldv(fb + 6, g);
+    }
+    const ddv3 r0 = A.lddv(F_X0R, g), v0 = A.lddv(F_X0V, g);
+    A.stddv(F_R, g, add(r0, scale(mkdd(dt), cv)));
+    A.stddv(F_V, g, add(v0, mkddv(dt * ca)));
+    const double t = d.t0[d.bsys[g]] + tadd;
+    A.st(F_GM, g, A.ld(F_GM0, g) + A.ld(F_DGM, g) * t);
+    A.st(F_EXJ2, g, A.ld(F_DJ2, g) * t);
+}
+
+__device__ __forceinline__ void stage_rotational(const Acc &A, int g, int k, double dt, double dt_k) {
+    if (k == 1) {
         A.stddv(F_X0GL, g, A.lddv(F_GL, g));
         A.stddm(F_X0S, g, A.lddm(F_S, g));
         A.stv(F_X0W, g, A.ldv(F_W, g));
@@ -1508,8 +1540,6 @@ __device__ __forceinline__ void stage_body_inl(const View &d, const Acc &A, int 
     const v3 w_now = A.ldv(F_W, g), tq_now = A.ldv(F_DTQ, g);
     {
         const int fb = F_STAGE + (k - 1) * kStageFields;
-        A.stddv(fb, g, v);
-        A.stv(fb + 6, g, (A.ldv(F_GACC, g) + A.ldv(F_DACC, g)) + A.ldv(F_NACC, g));
         A.stv(fb + 9, g, w_now);
         A.stv(fb + 12, g, tq_now);
     }
@@ -1529,24 +1559,18 @@ __device__ __forceinline__ void stage_body_inl(const View &d, const Acc &A, int 
         egrad = cross((t1 + cross(w, irtq)) + ir * ((cross(j, irtq) - 2 * cross(w, tq)) + tqdot), w);
         A.stv(F_EGRAD, g, egrad);
     }
-    ddv3 cv = mkddv(v3{0, 0, 0});
-    v3 ca{0, 0, 0}, cw{0, 0, 0}, ct{0, 0, 0};
+    v3 cw{0, 0, 0}, ct{0, 0, 0};
     const int base = k * (k - 1) / 2;
     for (int j = 0; j < k; ++j) {
         const double2 c = c_rk[base + j];
         if (c.x == 0) continue;
-        const dd ckj{c.x, c.y};
         const double fc = c.x + c.y;
         const int fb = F_STAGE + j * kStageFields;
-        cv = add(cv, scale(ckj, A.lddv(fb, g)));
-        ca += fc * A.ldv(fb + 6, g);
         cw += fc * A.ldv(fb + 9, g);
         ct += fc * A.ldv(fb + 12, g);
     }
-    const ddv3 r0 = A.lddv(F_X0R, g), v0 = A.lddv(F_X0V, g), gl0 = A.lddv(F_X0GL, g);
+    const ddv3 gl0 = A.lddv(F_X0GL, g);
     const v3 w0 = A.ldv(F_X0W, g);
-    A.stddv(F_R, g, add(r0, scale(mkdd(dt), cv)));
-    A.stddv(F_V, g, add(v0, mkddv(dt * ca)));
     const v3 w = cw / dt_k;
     ddm3 sm = A.lddm(F_X0S, g);
     const v3 dw = (2.0 / 3) * (w - w0);
@@ -1559,9 +1583,11 @@ __device__ __forceinline__ void stage_body_inl(const View &d, const Acc &A, int 
     }
     A.stddm(F_S, g, sm);
     A.stv(F_W, g, w);
-    const double t = d.t0[d.bsys[g]] + tadd;
-    A.st(F_GM, g, A.ld(F_GM0, g) + A.ld(F_DGM, g) * t);
-    A.st(F_EXJ2, g, A.ld(F_DJ2, g) * t);
+}
+
+__device__ __forceinline__ void stage_body_inl(const View &d, const Acc &A, int g, int k, double dt, double dt_k, double tadd) {
+    stage_translational(d, A, g, k, dt, tadd);
+    stage_rotational(A, g, k, dt, dt_k);
 }
 
 // the persistent small-system kernel keeps the stage update out of line
@@ -1572,13 +1598,18 @@ __device__ __noinline__ void stage_body(const View &d, const Acc &A, int g, int 
 // istep != nullptr: the step index comes from device memory (replayed graphs)
 // and the stage time offset dt * (dt_k + i_step) is formed here, with the
 // same two operations as on the host.
+// Workgroups [0, nblk) take the translational half, [nblk, 2 nblk) the
+// rotational half of the same bodies.
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_stage(View d, int k, double dt, double dt_k, double tadd,
                                               const int64_t *__restrict__ istep) {
     const Acc A{d.F, d.nbp};
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nblk = (d.nb + 63) / 64;
+    const bool rot = (int)blockIdx.x >= nblk;
+    const int g = ((int)blockIdx.x - (rot ? nblk : 0)) * 64 + (int)threadIdx.x;
     if (g >= d.nb) return;
     if (istep) tadd = dt * (dt_k + (double)*istep);
-    stage_body_inl(d, A, g, k, dt, dt_k, tadd);
+    if (rot) stage_rotational(A, g, k, dt, dt_k);
+    else stage_translational(d, A, g, k, dt, tadd);
 }
 
 __global__ void k_step_count(int64_t *istep) {
@@ -1979,7 +2010,7 @@ void launch_np(const View &d, int nrows, int max_n, int maxT, const int2 *recv_b
     if (nrecv) hipLaunchKernelGGL(k_np_recv, dim3(nrecv), dim3(kBS), 0, st, d, recv_blk, maxT);
 }
 void launch_stage(const View &d, int k, double dt, double dt_k, double tadd, const int64_t *istep, hipStream_t st) {
-    hipLaunchKernelGGL(k_stage, dim3((d.nb + 63) / 64), dim3(64), 0, st, d, k, dt, dt_k, tadd, istep);
+    hipLaunchKernelGGL(k_stage, dim3(2 * ((d.nb + 63) / 64)), dim3(64), 0, st, d, k, dt, dt_k, tadd, istep);
 }
 void launch_step_count(int64_t *istep, hipStream_t st) {
     hipLaunchKernelGGL(k_step_count, dim3(1), dim3(64), 0, st, istep);
