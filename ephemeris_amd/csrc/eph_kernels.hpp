@@ -69,7 +69,7 @@ void launch_main_serial(const View &d, const int2 *blk, int nblk, hipStream_t st
 void launch_main_wpb(const View &d, const int2 *work, int nwork, int max_n, bool fused_np, hipStream_t st);
 void launch_np_field(const View &d, int nrows, int max_n, hipStream_t st);
 constexpr int kNpFusedMaxN_h = 512, kNpFusedMaxItems_h = 64;
-// one workgroup of up to 8 wavefronts per body (partner chunks evaluated in
+// one workgroup of up to 4 wavefronts per body (partner chunks evaluated in
 // parallel, summed in order); max_n = largest system of the batch
 // np_rows > 0: the launch also evaluates the non-point item fields
 // (launch_np_field) in extra workgroups
