@@ -796,11 +796,20 @@ int regroup_batch(Batch *old, const std::vector<int> &keep, int64_t n_new, const
 
 // ---- test particles -------------------------------------------------------------
 Particles::~Particles() {
+    if (pstream) (void)hipStreamSynchronize(pstream);
+    for (int b = 0; b < 2; ++b) {
+        if (ev_rec[b]) (void)hipEventDestroy(ev_rec[b]);
+        if (ev_free[b]) (void)hipEventDestroy(ev_free[b]);
+    }
+    if (pstream) (void)hipStreamDestroy(pstream);
     if (P) (void)hipFree(P);
     if (rec) (void)hipFree(rec);
 }
 
-static constexpr int64_t kRecChunk = 128;      // steps of field records per chunk
+// Steps per chunk of field records.  The field computes chunk c + 1 while
+// the particles advance through chunk c, so a chunk of one step hides all
+// but the first step of the field's integration.
+static constexpr int64_t kRecChunk = 1;
 
 // The partner data (JMain) each body of the one-system batch `field`
 // presents at its current state -- what the last force evaluation left --
@@ -861,8 +870,14 @@ int create_particles(Batch *field, int64_t n, const double *r, const double *v, 
     if (hipMemcpy(ps->P, host.data(), sizeof(double) * host.size(), hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail(EPH_EDEVICE, "particle upload failed"));
     ps->rec_steps = kRecChunk;
-    const size_t rec_bytes = sizeof(JMain) * kSmallMax * (size_t)(25 * kRecChunk);
+    const size_t rec_bytes = 2 * sizeof(JMain) * kSmallMax * (size_t)(25 * kRecChunk);
     if (hipMalloc(&ps->rec, rec_bytes) != hipSuccess) return bail(fail(EPH_ENOMEM, "field record allocation failed"));
+    if (hipStreamCreateWithFlags(&ps->pstream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(EPH_EDEVICE, "hipStreamCreate failed"));
+    for (int b = 0; b < 2; ++b)
+        if (hipEventCreateWithFlags(&ps->ev_rec[b], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&ps->ev_free[b], hipEventDisableTiming) != hipSuccess)
+            return bail(fail(EPH_EDEVICE, "hipEventCreate failed"));
     {
         int rc = field_epoch_record(field, ps->rec);
         if (rc) return bail(rc);
@@ -874,22 +889,48 @@ int create_particles(Batch *field, int64_t n, const double *r, const double *v, 
     return EPH_OK;
 }
 
-namespace {
-struct PCtx { Particles *ps; double dt; };
-void particles_after_chunk(void *vctx, int64_t first, int64_t steps) {
-    (void)first;
-    PCtx *c = (PCtx *)vctx;
-    Particles *ps = c->ps;
-    PView pv{ps->P, ps->npp, (int)ps->np, ps->rec, 1, (int)ps->field->nb, ps->R < 0 ? ps->R * ps->R : 0.0};
-    Scope sc(ps->field->profiler, 6, ps->field->stream);
-    launch_particles(pv, (int)steps, c->dt, stage_dtk(), ps->field->stream);
-}
-}  // namespace
-
+// The field (k_small, on the field's stream) writes the records of chunk c
+// into buffer c % 2; the particles (on their own stream) replay them once
+// that launch is done; the field reuses a buffer once the particle launch
+// that read it is done.  Chunking does not change any result: the particle
+// state is stored between launches in full.
 int integrate_particles(Particles *ps, double dt, int64_t n_step) {
     if (n_step < 0) return fail(EPH_EINVAL, "particles integrate forward only");
-    PCtx ctx{ps, dt};
-    return ps->field->integrate_small(dt, n_step, ps->rec, ps->rec_steps, particles_after_chunk, &ctx);
+    if (n_step == 0) return EPH_OK;
+    Batch *f = ps->field;
+    std::vector<double> t0(f->n_sys);
+    for (int64_t s = 0; s < f->n_sys; ++s) t0[s] = todbl(f->t_eph[s]);
+    HIP_CHECK(hipMemcpyAsync(f->d_t0, t0.data(), sizeof(double) * f->n_sys, hipMemcpyHostToDevice, f->stream));
+    const View v = f->view(0);
+    const DtK dtk = stage_dtk();
+    launch_extrema(v, 0, f->stream);
+    const size_t buf = (size_t)25 * ps->rec_steps * kSmallMax;   // records per buffer
+    const double sR2 = ps->R < 0 ? ps->R * ps->R : 0.0;
+    int64_t c = 0;
+    for (int64_t s0 = 0; s0 < n_step; s0 += ps->rec_steps, ++c) {
+        const int64_t ns = std::min(ps->rec_steps, n_step - s0);
+        const int b = (int)(c & 1);
+        JMain *rec = ps->rec + b * buf;
+        if (c >= 2) HIP_CHECK(hipStreamWaitEvent(f->stream, ps->ev_free[b], 0));
+        {
+            Scope sc(f->profiler, 5, f->stream);
+            launch_small(v, f->d_small_ids, (int)f->small_ids.size(), (int)ns, dt, dtk, s0, rec, f->stream);
+        }
+        HIP_CHECK(hipEventRecord(ps->ev_rec[b], f->stream));
+        HIP_CHECK(hipStreamWaitEvent(ps->pstream, ps->ev_rec[b], 0));
+        {
+            PView pv{ps->P, ps->npp, (int)ps->np, rec, 1, (int)f->nb, sR2};
+            Scope sc(f->profiler, 6, ps->pstream);
+            launch_particles(pv, (int)ns, dt, dtk, ps->pstream);
+        }
+        HIP_CHECK(hipEventRecord(ps->ev_free[b], ps->pstream));
+    }
+    HIP_CHECK(hipStreamWaitEvent(f->stream, ps->ev_free[(c - 1) & 1], 0));
+    launch_extrema(v, 1, f->stream);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(f->stream));
+    for (int64_t s = 0; s < f->n_sys; ++s) f->t_eph[s] = host_add(f->t_eph[s], mkdd(dt * (double)n_step));
+    return EPH_OK;
 }
 
 int download_particles(Particles *ps, double *r, double *v, double *extrema) {

@@ -131,9 +131,13 @@ struct Particles {
     int64_t np = 0, npp = 0;
     double GM = 0, R = 0;
     double *P = nullptr;
-    JMain *rec = nullptr;
+    JMain *rec = nullptr;        // two buffers of field records, rec_steps steps each
     int64_t rec_steps = 0;       // steps recorded per chunk
     int *d_sys_ids = nullptr;
+    // particle launches run on their own stream (default priority), one
+    // chunk behind the field's launches on the field's stream
+    hipStream_t pstream = nullptr;
+    hipEvent_t ev_rec[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
     ~Particles();
 };
 
