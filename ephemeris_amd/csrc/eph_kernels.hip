@@ -482,11 +482,9 @@ __global__ __launch_bounds__(BS) void k_deform_light(View d, const int4 *__restr
 // subtracted terms stored negated.  The self pair and partners past the end
 // add +0, an exact identity.
 constexpr int kDefK4Pitch = 13;
-// fields of a deform partner: r (hi, lo), sR2, GM; then v (hi, lo) and the
-// tide delay factor for constant-time-lag bodies
-constexpr int kJDefF = 8, kJDefDampedF = 15;
-__constant__ int c_jdef_field[kJDefDampedF] = {F_R + 0, F_R + 1, F_R + 2, F_R + 3, F_R + 4, F_R + 5, F_SR2, F_GM,
-                                               F_V + 0, F_V + 1, F_V + 2, F_V + 3, F_V + 4, F_V + 5, F_TDF};
+// fields of a deform partner: r (hi, lo), sR2, GM
+constexpr int kJDefF = 8;
+__constant__ int c_jdef_field[kJDefF] = {F_R + 0, F_R + 1, F_R + 2, F_R + 3, F_R + 4, F_R + 5, F_SR2, F_GM};
 
 __global__ __launch_bounds__(64) void k_deform_k4(View d, const int4 *__restrict__ blk, int prepare, NpTail np) {
     if (np_tail(d, np)) return;

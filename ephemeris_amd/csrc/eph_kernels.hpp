@@ -62,10 +62,12 @@ void launch_deform_k4(const View &d, const int4 *blk16, int nblk, int prepare, i
 // four lanes per body; blk16 = (system, first body) of every 16 bodies
 void launch_main_k4(const View &d, const int2 *blk16, int nblk, hipStream_t st);
 void launch_main_serial(const View &d, const int2 *blk, int nblk, hipStream_t st);
-// latency path, one wavefront per body; work = {system, body index in system}
+// latency path, one workgroup of 1-4 wavefronts per body; work = {system,
+// body index in system}
 // fused_np: the receiver and own-item sums of the non-point pass run inside
-// each body's wavefront, after launch_np_field (every thread-pool-order
-// system has at most 512 bodies and 64 non-point items)
+// each body's workgroup, after the item fields are formed (launch_np_field or
+// the extra workgroups of a deform launch; every thread-pool-order system
+// has at most 512 bodies and 64 non-point items)
 void launch_main_wpb(const View &d, const int2 *work, int nwork, int max_n, bool fused_np, hipStream_t st);
 void launch_np_field(const View &d, int nrows, int max_n, hipStream_t st);
 constexpr int kNpFusedMaxN_h = 512, kNpFusedMaxItems_h = 64;
