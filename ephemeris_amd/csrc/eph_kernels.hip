@@ -1436,13 +1436,14 @@ __global__ __launch_bounds__(kBS) void k_np_recv(View d, const int2 *__restrict_
     const int s = blk[blockIdx.x].x, i0 = blk[blockIdx.x].y;
     const SysMeta sm = d.sys[s];
     const int T = sm.T, nit = sm.nitem, n = sm.n;
-    __shared__ int s_body[kRecvStaged], s_an[kRecvStaged];
+    __shared__ int s_body[kRecvStaged], s_an[kRecvStaged], s_kind[kRecvStaged];
     __shared__ double s_gm[kRecvStaged];
     const bool staged = nit <= kRecvStaged;
     if (staged && (int)threadIdx.x < nit) {
         const NPItem it = d.items[sm.item0 + threadIdx.x];
         s_body[threadIdx.x] = it.body;
         s_an[threadIdx.x] = it.an_off;
+        s_kind[threadIdx.x] = it.kind;
         s_gm[threadIdx.x] = A.ld(F_GM, it.body);
     }
     __syncthreads();
@@ -1473,8 +1474,14 @@ __global__ __launch_bounds__(kBS) void k_np_recv(View d, const int2 *__restrict_
     // own items: idaccel (geopotential) then jdaccel (ring), mass.cpp:298-307
     for (int kind = 0; kind < 2; ++kind) {
         for (int q = 0; q < nit; ++q) {
-            const NPItem it = d.items[sm.item0 + q];
-            if (it.body != gj || it.kind != kind) continue;
+            int body, knd;
+            if (staged) {
+                body = s_body[q]; knd = s_kind[q];
+            } else {
+                const NPItem it = d.items[sm.item0 + q];
+                body = it.body; knd = it.kind;
+            }
+            if (body != gj || knd != kind) continue;
             v3 sa{0, 0, 0}, sq{0, 0, 0};
             for (int k = 0; k < T; ++k) {
                 const double *p = d.part + 6 * ((int64_t)(sm.item0 + q) * maxT + k);
