@@ -905,235 +905,35 @@ __global__ __launch_bounds__(64) void k_main_k4(View d, const int2 *__restrict__
 }
 
 // ---------------------------------------------------------------------------
-// latency path: one wavefront per body.  Each lane evaluates one partner of
-// a 64-partner tile and leaves that pair's terms in LDS; then one lane per
-// accumulator adds them in partner order, so every sum is formed exactly as
-// in the in-order loop above while a 504-body system occupies 504 waves.
+// latency path: one workgroup of 1-4 wavefronts per body.  Each lane
+// evaluates one partner of a 64-partner tile and leaves that pair's terms in
+// LDS; then one lane per accumulator adds them in partner order, so every
+// sum is formed exactly as in the in-order loop above.
 //
-// The accumulating lanes run one common, branch-free instruction stream:
-// every lane adds kWpbSteps terms per partner (its own chain's terms, padded
-// with +0), and the self pair and partners past the end contribute +0.  A
-// sum that starts at +0 never holds -0 in round-to-nearest, so adding +0 is
-// an exact identity; a - b is a + (-b) exactly, so subtracted terms are
+// gaccel (+G per partner) is one chain per component over the tile's G rows.
+// daccel and dtorque take up to five additions per partner, in the reference
+// order of the pair (mass.cpp:147-155, RELATIVITY .. RADIATION_PRESSURE of
+// mass.impl); slot k is one addition for each of the six lanes:
+//   slot 0  +dg (own quadrupole)         +r x dg
+//   slot 1  +gj (partner's quadrupole)    .
+//   slot 2  -LT (own spin)               -LT torque
+//   slot 3  +LT (partner's spin)          .
+//   slot 4  +radiation                    .
+// ('.' adds +0).  The lane that evaluates a partner stores only the slots
+// with a non-zero term, as consecutive six-value entries: a sum that starts
+// at +0 never holds -0 in round-to-nearest, so adding a zero of either sign
+// is an exact identity and leaving it out changes nothing (most partners
+// have no quadrupole, no spin or no luminosity, so the dependent chain that
+// bounds this kernel is shorter).  The self pair and partners past the end
+// store nothing (G: +0).  a - b is a + (-b) exactly, so subtracted terms are
 // stored negated.  The running maxima do not depend on order (ties are equal
 // values, NaN wins either way) and are reduced across the wave at the end.
 // ---------------------------------------------------------------------------
-constexpr int kWpbRows = 25;          // 24 term rows + one row of zeros
-constexpr int kWpbPitch = 65;         // row pitch in doubles: rows start in different LDS banks
-constexpr int kWpbSteps = 5;
-constexpr int kWpbZero = kWpbRows - 1;
-
-// row of the k-th term added by accumulator lane `lane` (main force,
-// thread-pool order, mass.cpp:147-155): lanes 0-2 gaccel (+G), 3-5 daccel
-// (+own quadrupole, +partner's quadrupole, -LT own spin, +LT partner's spin,
-// +radiation), 6-8 dtorque (+r x dg, -LT torque)
-// Term rows: G 0-2, dg 3-5, r x dg 6-8, gj 9-11, -LT torque 12-14,
-// -LT own spin 15-17, LT partner's spin 18-20, radiation 21-23, zeros 24.
-// With an odd row pitch, the nine rows the accumulating lanes read at one
-// step are distinct modulo 16, so the reads hit distinct LDS banks.
-__device__ __forceinline__ int wpb_main_row(int lane, int k) {
-    const int c = lane % 3;
-    if (lane < 3) return k == 0 ? c : kWpbZero;
-    if (lane < 6) {
-        constexpr int base[kWpbSteps] = {3, 9, 15, 18, 21};     // dg, gj, -lt1, lt3, rad
-        return base[k] + c;
-    }
-    if (lane < 9) return k == 0 ? 6 + c : (k == 1 ? 12 + c : kWpbZero);
-    return kWpbZero;
-}
-
-__device__ __forceinline__ double wave_checked_max(double x) {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) x = checked_max_into(x, __shfl_xor(x, m, 64));
-    return x;
-}
-
-constexpr int kNpFusedMaxN = 512;     // systems up to this size take the fused non-point pass
-constexpr int kNpFusedMaxItems = 64;
-
-// Non-point pass of body i inside its own wavefront, after k_np_field has
-// left every item's field at every partner in d.an (k_np_item and k_np_recv
-// for one receiver): receiver terms in round order, then the sums of i's own
-// items over partner slices, then finalisation.  `acc` of lanes 3-5 holds
-// daccel, of lanes 6-8 dtorque; `ga` is gaccel.
-// pq: LDS scratch of 6 * n doubles (the pair-term tiles, no longer needed)
-__device__ __forceinline__ void wpb_nonpoint(const View &d, const Acc &A, const SysMeta &sm, int il, int gi,
-                                             double &acc, const v3 &ga, double *pq) {
-    const int lane = (int)threadIdx.x, nthreads = (int)blockDim.x;
-    const int n = sm.n, nit = sm.nitem, T = sm.T;
-    __shared__ double rf[kNpFusedMaxItems * 3];        // GM_p * field of item q at r_i (0 for i's own items)
-    __shared__ double part[kNpFusedMaxItems * 6];
-    __shared__ double fin[6];
-    __shared__ int own_q[2];
-    if (lane < 2) own_q[lane] = -1;
-    __syncthreads();
-    // (a) receiver side (mass.cpp:283-297): GM_p * an of every item at i
-    if (lane < nit) {
-        const NPItem it = d.items[sm.item0 + lane];
-        v3 f{0, 0, 0};
-        if (it.body != gi) {
-            const double *a = d.an + 3 * ((int64_t)it.an_off + il);
-            f = A.ld(F_GM, it.body) * v3{a[0], a[1], a[2]};
-        } else {
-            own_q[it.kind] = lane;
-        }
-        rf[3 * lane] = f.x; rf[3 * lane + 1] = f.y; rf[3 * lane + 2] = f.z;
-    }
-    __syncthreads();
-    if (lane >= 3 && lane < 6) {
-        const int c = lane - 3;
-        int sj = 0;
-        while ((sj + 1) * n / T <= il) ++sj;
-        for (int k = 0; k < T; ++k) {
-            int cc = sj - k;
-            if (cc < 0) cc += T;
-            const int first = cc * nit / T, last = (cc + 1) * nit / T;
-            for (int q = last - 1; q >= first; --q)
-                if (q != own_q[0] && q != own_q[1]) acc += rf[3 * q + c];
-        }
-    }
-    // (b) own items: geopotential (idaccel) first, then ring (jdaccel), mass.cpp:298-307
-    const ddv3 ri = A.lddv(F_R, gi);
-    for (int kind = 0; kind < 2; ++kind) {
-        const int q = own_q[kind];
-        if (q < 0) continue;
-        const NPItem it = d.items[sm.item0 + q];
-#pragma unroll 4
-        for (int jl = lane; jl < n; jl += nthreads) {     // pq[j]: GM_j * field at j, GM_j * r x field
-            v3 ta{0, 0, 0}, tt{0, 0, 0};
-            if (jl != il) {
-                const int gj = sm.off + jl;
-                const double *a = d.an + 3 * ((int64_t)it.an_off + jl);
-                const v3 an{a[0], a[1], a[2]};
-                const double GMj = A.ld(F_GM, gj);
-                ta = GMj * an;
-                tt = GMj * cross(diff(A.lddv(F_R, gj), ri), an);
-            }
-            double *o = pq + 6 * jl;
-            o[0] = ta.x; o[1] = ta.y; o[2] = ta.z; o[3] = tt.x; o[4] = tt.y; o[5] = tt.z;
-        }
-        __syncthreads();
-        if (lane < T) {                                    // round `lane` visits slice (chain + lane) mod T
-            int kk = it.chain + lane;
-            if (kk >= T) kk -= T;
-            const int j0 = kk * n / T, j1 = (kk + 1) * n / T;
-            v3 td{0, 0, 0}, tq{0, 0, 0};
-#pragma unroll 8
-            for (int jl = j0; jl < j1; ++jl) {             // the self pair holds +0: an exact identity
-                const double *o = pq + 6 * jl;
-                td -= v3{o[0], o[1], o[2]};
-                tq -= v3{o[3], o[4], o[5]};
-            }
-            double *w = part + 6 * lane;
-            w[0] = td.x; w[1] = td.y; w[2] = td.z; w[3] = tq.x; w[4] = tq.y; w[5] = tq.z;
-        }
-        __syncthreads();
-        if (lane >= 3 && lane < 9) {
-            const int col = lane - 3;                      // 0-2 acceleration, 3-5 torque
-            double sum = 0;
-            for (int k = 0; k < T; ++k) sum += part[6 * k + col];
-            const int f = (kind == 0 ? (col < 3 ? F_IDACC : F_IDTQ - 3) : (col < 3 ? F_JDACC : F_JDTQ - 3)) + col;
-            A.st(f, gi, sum);
-            acc += sum;
-        }
-        __syncthreads();
-    }
-    // (c) finalisation and ring correction (FINALIZE_RELATIVITY, RING_CORRECTION)
-    if (lane >= 3 && lane < 9) fin[lane - 3] = acc;
-    __syncthreads();
-    if (lane == 0) finalize_with(d, A, gi, ga, v3{fin[0], fin[1], fin[2]}, v3{fin[3], fin[4], fin[5]});
-}
-
-template <bool NP, int W>
-__global__ __launch_bounds__(64 * W) void k_main_wpb(View d, const int2 *__restrict__ work) {
-    static_assert(!NP || W * kWpbRows * kWpbPitch >= 6 * kNpFusedMaxN, "term tiles double as non-point scratch");
-    const Acc A{d.F, d.nbp};
-    const int s = work[blockIdx.x].x, il = work[blockIdx.x].y;
-    const SysMeta sm = d.sys[s];
-    const int n = sm.n, off = sm.off, gi = off + il;
-    const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
-    __shared__ double term[W * kWpbRows * kWpbPitch];
-    __shared__ double wmax[2 * W];
-    const IMain I = load_imain(A, gi);
-    double *my = term + wv * kWpbRows * kWpbPitch;
-    my[kWpbZero * kWpbPitch + lane] = 0.0;
-    int row[kWpbSteps];
-#pragma unroll
-    for (int k = 0; k < kWpbSteps; ++k) row[k] = wpb_main_row(lane, k) * kWpbPitch;
-    double acc = 0;
-    double mind = 0, maxi = 0;                             // lane-local maxima (>= +0: identities)
-    for (int t0 = 0; t0 < n; t0 += 64 * W) {
-        const int jl = t0 + 64 * wv + lane;
-        double t[kWpbZero];
-        if (jl < n && jl != il) {
-            JMain T;
-            load_jmain(A, T, off + jl);
-            const PairTerms pt = pair_terms(I, T);
-            const double v[kWpbZero] = {pt.G.x, pt.G.y, pt.G.z, pt.dg.x, pt.dg.y, pt.dg.z,
-                                        pt.rxdg.x, pt.rxdg.y, pt.rxdg.z, pt.gj.x, pt.gj.y, pt.gj.z,
-                                        -pt.lt2.x, -pt.lt2.y, -pt.lt2.z, -pt.lt1.x, -pt.lt1.y, -pt.lt1.z,
-                                        pt.lt3.x, pt.lt3.y, pt.lt3.z, pt.rad.x, pt.rad.y, pt.rad.z};
-#pragma unroll
-            for (int q = 0; q < kWpbZero; ++q) t[q] = v[q];
-            mind = checked_max_into(mind, pt.rr);
-            maxi = checked_max_into(maxi, pt.tp_dg);
-        } else {
-#pragma unroll
-            for (int q = 0; q < kWpbZero; ++q) t[q] = 0.0;
-        }
-        if (t0 + 64 * wv < n) {
-#pragma unroll
-            for (int q = 0; q < kWpbZero; ++q) my[q * kWpbPitch + lane] = t[q];
-        }
-        __syncthreads();
-        if (wv == 0 && lane < 9) {
-            const int nw = min(W, (n - t0 + 63) / 64);
-            for (int u = 0; u < nw; ++u) {
-                const double *tile = term + u * kWpbRows * kWpbPitch;
-#pragma unroll 8
-                for (int q = 0; q < 64; ++q)
-#pragma unroll
-                    for (int k = 0; k < kWpbSteps; ++k) acc += tile[row[k] + q];
-            }
-        }
-        __syncthreads();
-    }
-    mind = wave_checked_max(mind);
-    maxi = wave_checked_max(maxi);
-    if (lane == 0) { wmax[wv] = mind; wmax[W + wv] = maxi; }
-    __syncthreads();
-    if (wv == 0) {
-        for (int u = 1; u < W; ++u) {
-            mind = checked_max_into(mind, wmax[u]);
-            maxi = checked_max_into(maxi, wmax[W + u]);
-        }
-        mind = checked_max_into(A.ld(F_MIND, gi), mind);
-        maxi = checked_max_into(A.ld(F_MAXI, gi), maxi);
-        if (lane < 3) A.st(F_GACC + lane, gi, acc);
-        else if (lane == 9) A.st(F_MIND, gi, mind);
-        else if (lane == 10) A.st(F_MAXI, gi, maxi);
-    }
-    if (NP && sm.nitem > 0) {
-        __shared__ double gsh[3];
-        if (threadIdx.x < 3) gsh[threadIdx.x] = acc;
-        __syncthreads();
-        wpb_nonpoint(d, A, sm, il, gi, acc, v3{gsh[0], gsh[1], gsh[2]}, term);
-        return;
-    }
-    if (wv != 0) return;
-    if (lane >= 3 && lane < 6) A.st(F_DACC + lane - 3, gi, acc);
-    else if (lane >= 6 && lane < 9) A.st(F_DTQ + lane - 6, gi, acc);
-    if (sm.nitem == 0 && lane == 11) {                     // FINALIZE_RELATIVITY
-        A.st(F_PHI, gi, I.phi * kC2);
-        A.stv(F_NACC, gi, A.ldv(F_NACCC, gi) * kC2);
-    }
-}
-
-// deform + spin fixed point + PREPARE_RELATIVITY, one wavefront per body.
-// Accumulators: lane 0 phi, 1-3 naccel, 4-12 the tidal matrix; the
-// subtracted terms (phi, tidal matrix) are stored negated and the self pair
-// contributes +0, as in k_main_wpb.
-constexpr int kWpbDefPitch = 65;
+constexpr int kWpbPitch = 65;                        // G row pitch: rows start in different LDS banks
+constexpr int kWpbSlots = 5;
+constexpr int kWpbEnt = 6;                           // doubles per entry
+constexpr int kWpbEntMax = 64 * kWpbSlots + 16;      // entries of one tile, padded to a multiple of 16
+constexpr int kWpbTile = 3 * kWpbPitch + kWpbEnt * kWpbEntMax;
 
 // acc += row[0] + ... + row[63], one dependent addition at a time in index
 // order; the reads of the next eight values are issued before this eight
@@ -1156,6 +956,336 @@ __device__ __forceinline__ void chain_add64(const double *row, double &acc) {
         for (int i = 0; i < 8; ++i) acc += b1[i];
     }
 }
+// acc += p[0] + p[S] + ... + p[(cnt - 1) * S], one dependent addition at a
+// time in index order (cnt a multiple of 16, uniform across the wave); the
+// reads of the next sixteen are issued before this sixteen are added.
+template <int S>
+__device__ __forceinline__ void chain_add_strided(const double *p, int cnt, double &acc) {
+    if (cnt <= 0) return;
+    double b0[16], b1[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) b0[i] = p[i * S];
+    for (int e = 0; e < cnt; e += 32) {
+        if (e + 16 < cnt) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) b1[i] = p[(e + 16 + i) * S];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc += b0[i];
+        if (e + 16 >= cnt) break;
+        if (e + 32 < cnt) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) b0[i] = p[(e + 32 + i) * S];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc += b1[i];
+    }
+}
+
+__device__ __forceinline__ double wave_checked_max(double x) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) x = checked_max_into(x, __shfl_xor(x, m, 64));
+    return x;
+}
+
+constexpr int kNpFusedMaxN = 512;     // systems up to this size take the fused non-point pass
+constexpr int kNpFusedMaxItems = 64;
+
+// Non-point pass of body i inside its own workgroup (mass.cpp:283-307),
+// after k_np_field has left every item's field at every partner in d.an.
+// Its inputs are gathered by spare wavefronts while wave 0 adds the last
+// tile's chains, so that only the ordered additions follow the pair loop.
+//
+// Receiver side, lane q < nitem: GM of item q's body times its field at i
+// (zero for i's own items, which the receiver sum skips).
+__device__ __forceinline__ v3 np_recv_term(const View &d, const Acc &A, const SysMeta &sm, int il, int gi, int q,
+                                           int &own_kind) {
+    const NPItem it = d.items[sm.item0 + q];
+    own_kind = it.body == gi ? it.kind : -1;
+    if (it.body == gi) return v3{0, 0, 0};
+    const double *a = d.an + 3 * ((int64_t)it.an_off + il);
+    return A.ld(F_GM, it.body) * v3{a[0], a[1], a[2]};
+}
+// FINALIZE_RELATIVITY of a body with non-point items and the inputs of its
+// RING_CORRECTION (mass.impl:133-152; finalize_with): phi and naccel are
+// final already; ring: {GM_ratio, rmGL / (rmGL + GL_R2 * R2), mGL, naccel}
+__device__ __forceinline__ void np_finalize_pre(const View &d, const Acc &A, int gi, double phic, double *ring8) {
+    A.st(F_PHI, gi, phic * kC2);
+    const v3 nacc = A.ldv(F_NACCC, gi) * kC2;
+    A.stv(F_NACC, gi, nacc);
+    const int rg = d.np_ring[gi];
+    if (rg >= 0) {
+        const eph_ring &mr = d.rings[rg];
+        v3 mGL = todbl(A.lddv(F_GL, gi));
+        const double rmGL = normalize(mGL);
+        ring8[0] = mr.GM_ratio;
+        ring8[1] = rmGL / (rmGL + mr.GL_R2 * A.ld(F_R2, gi));
+        ring8[2] = mGL.x; ring8[3] = mGL.y; ring8[4] = mGL.z;
+        ring8[5] = nacc.x; ring8[6] = nacc.y; ring8[7] = nacc.z;
+    }
+}
+// LDS scratch of the non-point pass (in the pair-term tiles, free by then):
+// rf, ring inputs, two own items' round sums, then per own item the
+// products of every partner (6 doubles each)
+constexpr int kNpHead = 15 * kNpFusedMaxItems + 8;
+constexpr int kNpScratch = kNpHead + 6 * kNpFusedMaxN;
+
+template <bool NP, int W>
+__global__ __launch_bounds__(64 * W) void k_main_wpb(View d, const int2 *__restrict__ work) {
+    static_assert(!NP || W * kWpbTile >= kNpScratch, "term tiles double as non-point scratch");
+    const Acc A{d.F, d.nbp};
+    const int s = work[blockIdx.x].x, il = work[blockIdx.x].y;
+    const SysMeta sm = d.sys[s];
+    const int n = sm.n, off = sm.off, gi = off + il;
+    const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
+    __shared__ double term[W * kWpbTile];
+    __shared__ int count[W];
+    __shared__ double wmax[2 * W];
+    __shared__ double gsum[3];
+    __shared__ int own_s[2];
+    const IMain I = load_imain(A, gi);
+    double mind0 = 0, maxi0 = 0;
+    if (wv == 0) { mind0 = A.ld(F_MIND, gi); maxi0 = A.ld(F_MAXI, gi); }
+    double *myg = term + wv * kWpbTile, *mye = myg + 3 * kWpbPitch;
+    double acc = 0, gacc = 0;                              // lanes 0-5 of wave 0: daccel, dtorque; gaccel
+    double mind = 0, maxi = 0;                             // lane-local maxima (>= +0: identities)
+    constexpr int kGWave = W > 1 ? 1 : 0;                  // the wave that adds the G chains
+    // the waves that gather the non-point inputs during the last tile's chains
+    constexpr int kHA = W > 2 ? 2 : W - 1, kHB = W > 3 ? 3 : kHA;
+    const bool np = NP && sm.nitem > 0;
+    v3 rfv{0, 0, 0}, tdA{0, 0, 0}, tqA{0, 0, 0}, tdB{0, 0, 0}, tqB{0, 0, 0};
+    double ring8[8];
+    int own0 = -1, own1 = -1;
+    for (int t0 = 0; t0 < n; t0 += 64 * W) {
+        const int jl = t0 + 64 * wv + lane;
+        unsigned keep = 0;
+        v3 e[2 * kWpbSlots];
+        v3 G{0, 0, 0};
+        if (jl < n && jl != il) {
+            JMain T;
+            load_jmain(A, T, off + jl);
+            const PairTerms pt = pair_terms(I, T);
+            G = pt.G;
+            e[0] = pt.dg; e[1] = pt.rxdg;
+            e[2] = pt.gj; e[3] = v3{0, 0, 0};
+            e[4] = -pt.lt1; e[5] = -pt.lt2;
+            e[6] = pt.lt3; e[7] = v3{0, 0, 0};
+            e[8] = pt.rad; e[9] = v3{0, 0, 0};
+            auto nz = [](const v3 &a) { return a.x != 0 || a.y != 0 || a.z != 0; };
+#pragma unroll
+            for (int k = 0; k < kWpbSlots; ++k)
+                if (nz(e[2 * k]) || nz(e[2 * k + 1])) keep |= 1u << k;
+            mind = checked_max_into(mind, pt.rr);
+            maxi = checked_max_into(maxi, pt.tp_dg);
+        }
+        myg[lane] = G.x; myg[kWpbPitch + lane] = G.y; myg[2 * kWpbPitch + lane] = G.z;
+        // this tile's entries: partner order, slot order within a partner
+        int pos = 0, total = 0;
+#pragma unroll
+        for (int k = 0; k < kWpbSlots; ++k) {
+            const uint64_t b = __ballot((keep >> k) & 1u);
+            pos += __popcll(b & ((1ull << lane) - 1));
+            total += __popcll(b);
+        }
+#pragma unroll
+        for (int k = 0; k < kWpbSlots; ++k)
+            if ((keep >> k) & 1u) {
+                double *o = mye + kWpbEnt * pos++;
+                o[0] = e[2 * k].x; o[1] = e[2 * k].y; o[2] = e[2 * k].z;
+                o[3] = e[2 * k + 1].x; o[4] = e[2 * k + 1].y; o[5] = e[2 * k + 1].z;
+            }
+        const int padded = (total + 15) & ~15;
+        if (lane < padded - total) {
+            double *o = mye + kWpbEnt * (total + lane);
+#pragma unroll
+            for (int c = 0; c < kWpbEnt; ++c) o[c] = 0.0;
+        }
+        if (lane == 0) count[wv] = padded;
+        __syncthreads();
+        if (wv == 0 && lane < 6) {
+            for (int u = 0; u < W; ++u)
+                chain_add_strided<kWpbEnt>(term + u * kWpbTile + 3 * kWpbPitch + lane,
+                                           __builtin_amdgcn_readfirstlane(count[u]), acc);
+        }
+        if (wv == kGWave && lane < 3) {
+            const int nw = min(W, (n - t0 + 63) / 64);
+            for (int u = 0; u < nw; ++u) chain_add64(term + u * kWpbTile + lane * kWpbPitch, gacc);
+        }
+        if (np && t0 + 64 * W >= n && wv == kHA) {
+            int kind = -1;
+            if (lane < sm.nitem) rfv = np_recv_term(d, A, sm, il, gi, lane, kind);
+            const uint64_t b0 = __ballot(kind == 0), b1 = __ballot(kind == 1);
+            own0 = b0 ? __ffsll((unsigned long long)b0) - 1 : -1;
+            own1 = b1 ? __ffsll((unsigned long long)b1) - 1 : -1;
+            if (lane == 0) np_finalize_pre(d, A, gi, I.phi, ring8);
+        }
+        __syncthreads();
+    }
+    // the non-point inputs into the (now free) tiles
+    double *rf = term, *ringS = rf + 3 * kNpFusedMaxItems, *partA = ringS + 8,
+           *partB = partA + 6 * kNpFusedMaxItems, *pq = term + kNpHead;
+    constexpr bool kBoth = W * kWpbTile >= kNpHead + 12 * kNpFusedMaxN;   // both own items' products at once
+    if (np && wv == kHA) {
+        if (lane < sm.nitem) { rf[3 * lane] = rfv.x; rf[3 * lane + 1] = rfv.y; rf[3 * lane + 2] = rfv.z; }
+        if (lane == 0) {
+            own_s[0] = own0; own_s[1] = own1;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) ringS[k] = ring8[k];
+        }
+    }
+    // gather the nine sums in lanes 0-8 of wave 0: gaccel, daccel, dtorque
+    if (wv == kGWave && lane < 3) gsum[lane] = gacc;
+    __syncthreads();
+    if (wv == 0) {
+        const double a6 = __shfl(acc, lane >= 3 ? lane - 3 : 0, 64);
+        acc = lane < 3 ? gsum[lane] : a6;
+    }
+    mind = wave_checked_max(mind);
+    maxi = wave_checked_max(maxi);
+    if (lane == 0) { wmax[wv] = mind; wmax[W + wv] = maxi; }
+    // own items (mass.cpp:298-307): the products GM_j * field and
+    // GM_j * (r_j - r_i) x field of every partner (+0 for the self pair), then
+    // round r < T subtracts those of partner slice (chain + r) mod T in order,
+    // one lane per round and component
+    if (np) {
+        const int T = sm.T;
+        const int tid = (int)threadIdx.x;
+        const int q0 = own_s[0], q1 = own_s[1];
+        for (int pass = 0; pass < (kBoth ? 1 : 2); ++pass) {
+            const bool do0 = q0 >= 0 && (kBoth || pass == 0), do1 = q1 >= 0 && (kBoth || pass == 1);
+            if (do0 || do1) {
+                const int off0 = do0 ? d.items[sm.item0 + q0].an_off : 0;
+                const int off1 = do1 ? d.items[sm.item0 + q1].an_off : 0;
+                double *p0 = pq, *p1 = pq + (kBoth ? 6 * kNpFusedMaxN : 0);
+                for (int jl = tid; jl < n; jl += 64 * W) {
+                    v3 a0{0, 0, 0}, t0v{0, 0, 0}, a1{0, 0, 0}, t1v{0, 0, 0};
+                    if (jl != il) {
+                        const int gj = off + jl;
+                        const double GMj = A.ld(F_GM, gj);
+                        const v3 rij = diff(A.lddv(F_R, gj), I.r);
+                        if (do0) {
+                            const double *a = d.an + 3 * ((int64_t)off0 + jl);
+                            const v3 an{a[0], a[1], a[2]};
+                            a0 = GMj * an;
+                            t0v = GMj * cross(rij, an);
+                        }
+                        if (do1) {
+                            const double *a = d.an + 3 * ((int64_t)off1 + jl);
+                            const v3 an{a[0], a[1], a[2]};
+                            a1 = GMj * an;
+                            t1v = GMj * cross(rij, an);
+                        }
+                    }
+                    if (do0) {
+                        double *o = p0 + 6 * jl;
+                        o[0] = a0.x; o[1] = a0.y; o[2] = a0.z; o[3] = t0v.x; o[4] = t0v.y; o[5] = t0v.z;
+                    }
+                    if (do1) {
+                        double *o = p1 + 6 * jl;
+                        o[0] = a1.x; o[1] = a1.y; o[2] = a1.z; o[3] = t1v.x; o[4] = t1v.y; o[5] = t1v.z;
+                    }
+                }
+            }
+            __syncthreads();
+            // item 0's rounds in wave 0, item 1's in wave 1 (both in wave 0 for one item at a time)
+            const int k2 = kBoth ? tid >> 6 : pass;
+            const int q = k2 == 0 ? q0 : q1;
+            if ((kBoth ? k2 < 2 : tid < 64) && q >= 0 && (kBoth || (k2 == 0 ? do0 : do1))) {
+                const double *pk = k2 == 0 ? pq : pq + (kBoth ? 6 * kNpFusedMaxN : 0);
+                const int chain = d.items[sm.item0 + q].chain;
+                double *part = k2 == 0 ? partA : partB;
+                for (int u = tid & 63; u < 6 * T; u += 64) {
+                    const int r = u / 6, c = u - 6 * r;
+                    int kk = chain + r;
+                    if (kk >= T) kk -= T;
+                    const int j0 = kk * n / T, j1 = (kk + 1) * n / T;
+                    double x = 0;
+#pragma unroll 8
+                    for (int jl = j0; jl < j1; ++jl) x -= pk[6 * jl + c];   // the self pair holds +0: an exact identity
+                    part[6 * r + c] = x;
+                }
+            }
+            if (!kBoth && pass == 0) __syncthreads();      // the products of item 1 overwrite item 0's
+        }
+    }
+    __syncthreads();
+    if (wv != 0) return;
+    for (int u = 1; u < W; ++u) {
+        mind = checked_max_into(mind, wmax[u]);
+        maxi = checked_max_into(maxi, wmax[W + u]);
+    }
+    mind = checked_max_into(mind0, mind);
+    maxi = checked_max_into(maxi0, maxi);
+    if (lane < 3) A.st(F_GACC + lane, gi, acc);
+    else if (lane == 9) A.st(F_MIND, gi, mind);
+    else if (lane == 10) A.st(F_MAXI, gi, maxi);
+    if (np) {
+        const int nit = sm.nitem, T = sm.T;
+        const int o0 = own_s[0], o1 = own_s[1];
+        // (a) receiver side (mass.cpp:283-297): rounds k = 0..T-1 visit item
+        // chunks sj, sj-1, ... (mod T), each from its last item down, where
+        // sj is the body's partner chunk: one cyclic descending pass over the
+        // items from the last item of chunk sj
+        if (lane >= 3 && lane < 6) {
+            const int c = lane - 3;
+            const int sj = ((il + 1) * T - 1) / n;
+            int q = (sj + 1) * nit / T - 1;
+            for (int m = 0; m < nit; ++m) {
+                if (q != o0 && q != o1) acc += rf[3 * q + c];
+                q = q == 0 ? nit - 1 : q - 1;
+            }
+        }
+        // (b) own items: geopotential (idaccel) first, then ring (jdaccel), mass.cpp:298-307
+        for (int kind = 0; kind < 2; ++kind) {
+            if ((kind == 0 ? o0 : o1) < 0) continue;
+            const double *part = kind == 0 ? partA : partB;
+            if (lane >= 3 && lane < 9) {
+                const int col = lane - 3;                  // 0-2 acceleration, 3-5 torque
+                double sum = 0;
+                for (int k = 0; k < T; ++k) sum += part[6 * k + col];
+                const int f = (kind == 0 ? (col < 3 ? F_IDACC : F_IDTQ - 3) : (col < 3 ? F_JDACC : F_JDTQ - 3)) + col;
+                A.st(f, gi, sum);
+                acc += sum;
+            }
+        }
+        // (c) daccel and dtorque with RING_CORRECTION (phi and naccel are written)
+        if (d.np_ring[gi] < 0) {
+            if (lane >= 3 && lane < 6) A.st(F_DACC + lane - 3, gi, acc);
+            else if (lane >= 6 && lane < 9) A.st(F_DTQ + lane - 6, gi, acc);
+        } else {
+            double v9[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) v9[k] = __shfl(acc, k, 64);
+            if (lane == 0) {
+                const v3 ga{v9[0], v9[1], v9[2]}, nacc{ringS[5], ringS[6], ringS[7]}, mGL{ringS[2], ringS[3], ringS[4]};
+                v3 da{v9[3], v9[4], v9[5]}, dq{v9[6], v9[7], v9[8]};
+                da = da - ringS[0] * ((ga + da) + nacc);
+                const v3 ptorque = dot(dq, mGL) * mGL;
+                v3 dtq = dq - ptorque;
+                dtq = ringS[1] * dtq;
+                dq = dtq + ptorque;
+                A.stv(F_DACC, gi, da);
+                A.stv(F_DTQ, gi, dq);
+            }
+        }
+        return;
+    }
+    if (lane >= 3 && lane < 6) A.st(F_DACC + lane - 3, gi, acc);
+    else if (lane >= 6 && lane < 9) A.st(F_DTQ + lane - 6, gi, acc);
+    if (sm.nitem == 0 && lane == 11) {                     // FINALIZE_RELATIVITY
+        A.st(F_PHI, gi, I.phi * kC2);
+        A.stv(F_NACC, gi, A.ldv(F_NACCC, gi) * kC2);
+    }
+}
+
+// deform + spin fixed point + PREPARE_RELATIVITY, one wavefront per body.
+// Accumulators: lane 0 phi, 1-3 naccel, 4-12 the tidal matrix; the
+// subtracted terms (phi, tidal matrix) are stored negated and the self pair
+// contributes +0, as in k_main_wpb.
+constexpr int kWpbDefPitch = 65;
+
 // W waves per body: in each round wave w evaluates partner chunk
 // (round * W + w); wave 0 then adds the W chunks in partner order.
 // TAIL: the launch carries the non-point item fields in extra workgroups
