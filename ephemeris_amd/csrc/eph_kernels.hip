@@ -1304,6 +1304,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 1 ?
     __shared__ double sums[13];
     __shared__ int again_s;
     __shared__ double wsh[3];
+    __shared__ double hsh[19];                             // harmonics and GL, gathered by wave 1
     const ddv3 ri = A.lddv(F_R, gi), vi = A.lddv(F_V, gi);
     const double sR2 = A.ld(F_SR2, gi), k2 = A.ld(F_K2, gi), td = A.ld(F_TD, gi);
     const bool damped = k2 != 0 && td != 0, ndelay = k2 != 0 && td == 0;
@@ -1382,6 +1383,20 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 1 ?
                 const int nw = min(W, (n - t0 + 63) / 64);
                 for (int u = 0; u < nw; ++u) chain_add64(term[u] + lane * kWpbDefPitch, acc);
             }
+            // the body's static harmonics in the world frame and GL (neither
+            // changes in this kernel before the spin step reads them), while
+            // wave 0 adds the first chains
+            if (W > 1 && first_pass && t0 == 0 && wv == 1 && lane == 0) {
+                Harmonics h;
+                h.load(A, gi);
+                const ddv3 gl = A.lddv(F_GL, gi);
+                hsh[0] = h.k2r; hsh[1] = h.scale; hsh[2] = h.R2; hsh[3] = h.A;
+                const double *c = &h.Cw.x.x;
+#pragma unroll
+                for (int q = 0; q < 9; ++q) hsh[4 + q] = c[q];
+                hsh[13] = gl.x.hi; hsh[14] = gl.x.lo; hsh[15] = gl.y.hi; hsh[16] = gl.y.lo;
+                hsh[17] = gl.z.hi; hsh[18] = gl.z.lo;
+            }
             __syncthreads();
         }
         __syncthreads();
@@ -1390,8 +1405,19 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 1 ?
         __syncthreads();
         if (threadIdx.x == 0) {
             const m3 Csum{v3{sums[4], sums[5], sums[6]}, v3{sums[7], sums[8], sums[9]}, v3{sums[10], sums[11], sums[12]}};
-            if (iters_left == 4) H.load(A, gi);
-            const ddv3 GL = A.lddv(F_GL, gi);
+            ddv3 GL;
+            if (W > 1) {
+                if (iters_left == 4) {
+                    H.k2r = hsh[0]; H.scale = hsh[1]; H.R2 = hsh[2]; H.A = hsh[3];
+                    double *c = &H.Cw.x.x;
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) c[q] = hsh[4 + q];
+                }
+                GL = ddv3{dd{hsh[13], hsh[14]}, dd{hsh[15], hsh[16]}, dd{hsh[17], hsh[18]}};
+            } else {
+                if (iters_left == 4) H.load(A, gi);
+                GL = A.lddv(F_GL, gi);
+            }
             bool brk;
             if (damped) {
                 H.apply(Csum, w, C, GI);
