@@ -3,11 +3,13 @@
 Workload (N=1): the 504-body full-model system (seeded, tests/golden/full504),
 RK12 at dt=300 s.  A step is one RK12 step = 25 force evaluations of the
 system; the timed region is K steps inside one eph_integrate call, with the
-state already resident in HBM.
+state already resident in HBM, launched from the host with HIP events around
+the main-force launches only (graph replay, the library default, is timed
+separately and reported under graph_replay).
 
 Prints one JSON line:
   value   FP64 pair-interactions/s = K * 25 * N*(N-1) / wall time
-  roofline  the dominant kernel class: algorithmic FP64 FLOP per launch /
+  roofline  the main-force kernel: algorithmic FP64 FLOP per launch /
             average launch time measured with HIP events on the library's
             stream over the timed region, against MI355X's FP64 vector peak
   cpu_baseline  the oracle (C restatement of the reference CPU path, threaded
@@ -226,7 +228,9 @@ def main():
     n = len(masses)
     g = GpuSystems(masses, None, gps, rings)
     g.integrate(DT, a.warmup)
-    # timed region: host-launched kernels with HIP events around every kernel class
+    # timed region: host-launched kernels, HIP events around the launches of
+    # the roofline kernel (each timed launch adds two event records)
+    g.profile_classes(["main_force_pool"])
     g.profile(True)
     g.profile_reset()
     t0 = time.perf_counter()
@@ -241,8 +245,15 @@ def main():
     t0 = time.perf_counter()
     g.integrate(DT, a.steps)
     el_graph = time.perf_counter() - t0
+    # device time of every kernel class, outside the timed regions
+    g.profile_classes(list(prof))
+    g.profile(True)
+    g.profile_reset()
+    g.integrate(DT, 4)
+    allprof = g.profile_read()
+    g.profile(False)
 
-    tot = {k: v[0] * v[1] for k, v in prof.items()}
+    tot = {k: v[0] * v[1] for k, v in allprof.items()}
     dominant = max(tot, key=tot.get)
     main_ms = prof["main_force_pool"][0]
     flop_main = n * (n - 1) * FLOP_MAIN_PAIR
@@ -253,8 +264,8 @@ def main():
                 "frac": achieved / FP64_VECTOR_PEAK_TFLOPS,
                 "traffic": profiled_traffic("void eph::k_main_wpb<true, 4>", n * 256), "traffic_unit": "bytes per launch",
                 "flop_per_launch": flop_main, "avg_launch_ms": main_ms,
-                "kernel_ms": {k: round(v[0], 5) for k, v in prof.items()},
-                "kernel_launches": {k: v[1] for k, v in prof.items()}}
+                "kernel_ms": {k: round(v[0], 5) for k, v in allprof.items() if v[1]},
+                "kernel_launches_timed_region": {k: v[1] for k, v in prof.items() if v[1]}}
     out = {"metric": "FP64 pair-interactions/s (504-body RK12)", "value": value,
            "unit": "pair-interactions/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
            "ms_per_step": el / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",

@@ -34,6 +34,7 @@ SIGNATURES = {
     "eph_profile_enable": (I, [P, I]),
     "eph_profile_read": (I, [P, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.POINTER(C.c_int64), I]),
     "eph_profile_reset": (I, [P]),
+    "eph_profile_classes": (I, [P, C.c_uint]),
     "eph_system_set_variant": (I, [P, I, I]),
     "eph_system_set_graph": (I, [P, I]),
     "eph_system_set_autotune": (I, [P, I]),

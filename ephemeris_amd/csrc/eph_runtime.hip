@@ -73,8 +73,8 @@ const char *const kKernelNames[kNumKernelClasses] = {
     "stage_update", "deform_spin", "main_force_pool", "main_force_serial", "nonpoint", "small_system_steps",
     "particle_steps"};
 
-void Profiler::begin(int cls, hipStream_t st) {
-    if (!enabled) return;
+bool Profiler::begin(int cls, hipStream_t st) {
+    if (!enabled || !((mask >> cls) & 1u)) return false;
     Rec r;
     r.cls = cls;
     if (free_ev.size() < 2) {
@@ -88,6 +88,7 @@ void Profiler::begin(int cls, hipStream_t st) {
     r.b = free_ev.back(); free_ev.pop_back();
     (void)hipEventRecord(r.a, st);
     open.push_back(r);
+    return true;
 }
 void Profiler::end(hipStream_t st) {
     if (!enabled || open.empty()) return;
@@ -118,8 +119,9 @@ Profiler::~Profiler() {
 struct Scope {
     Profiler &p;
     hipStream_t st;
-    Scope(Profiler &p_, int cls, hipStream_t s) : p(p_), st(s) { p.begin(cls, st); }
-    ~Scope() { p.end(st); }
+    bool on;
+    Scope(Profiler &p_, int cls, hipStream_t s) : p(p_), st(s), on(p_.begin(cls, s)) {}
+    ~Scope() { if (on) p.end(st); }
 };
 
 // ---- system ------------------------------------------------------------------
@@ -1108,6 +1110,13 @@ int eph_profile_enable(eph_system *sys, int enable) {
     if (!sys) return fail(EPH_EINVAL, "null system");
     B(sys)->profiler.collect();
     B(sys)->profiler.enabled = enable != 0;
+    return EPH_OK;
+}
+
+int eph_profile_classes(eph_system *sys, unsigned mask) {
+    if (!sys) return fail(EPH_EINVAL, "null system");
+    B(sys)->profiler.collect();
+    B(sys)->profiler.mask = mask;
     return EPH_OK;
 }
 

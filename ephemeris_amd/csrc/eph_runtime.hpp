@@ -27,11 +27,12 @@ extern const char *const kKernelNames[kNumKernelClasses];
 struct Profiler {
     struct Rec { int cls; hipEvent_t a, b; };
     bool enabled = false;
+    unsigned mask = ~0u;             // kernel classes timed while enabled
     std::vector<Rec> open, pending;
     std::vector<hipEvent_t> free_ev;
     double total_ms[kNumKernelClasses] = {};
     int64_t count[kNumKernelClasses] = {};
-    void begin(int cls, hipStream_t st);
+    bool begin(int cls, hipStream_t st);   // false: this class is not timed
     void end(hipStream_t st);
     void collect();
     void reset();

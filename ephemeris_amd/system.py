@@ -93,6 +93,14 @@ class GpuSystems:
     def profile(self, enable=True):
         _lib.check(_lib.lib().eph_profile_enable(self.h, int(enable)))
 
+    def profile_classes(self, names):
+        """Time only the named kernel classes (the names profile_read returns)."""
+        known = list(self.profile_read())
+        mask = 0
+        for nm in names:
+            mask |= 1 << known.index(nm)
+        _lib.check(_lib.lib().eph_profile_classes(self.h, mask))
+
     def profile_reset(self):
         _lib.check(_lib.lib().eph_profile_reset(self.h))
 

@@ -242,6 +242,10 @@ int eph_server_destroy(eph_server *srv);
  * HIP events on the library's stream since the last reset; names[] gets
  * static strings.  Returns the number of classes written. */
 int eph_profile_enable(eph_system *sys, int enable);
+/* Time only the classes whose bit (1 << index in the names of
+ * eph_profile_read) is set; all by default.  Each timed launch adds two
+ * event records to the stream. */
+int eph_profile_classes(eph_system *sys, unsigned mask);
 int eph_profile_read(eph_system *sys, const char **names, double *avg_ms, int64_t *launches, int cap);
 int eph_profile_reset(eph_system *sys);
 
