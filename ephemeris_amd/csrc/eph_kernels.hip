@@ -1305,6 +1305,10 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 1 ?
     __shared__ int again_s;
     __shared__ double wsh[3];
     __shared__ double hsh[19];                             // harmonics and GL, gathered by wave 1
+    // constant-time-lag bodies repeat the pair loop for each spin pass: with
+    // four wavefronts the spin-independent part of every pair is kept here
+    constexpr int kGeo = 10, kGeoN = 512;
+    __shared__ double geo[W == 4 ? kGeo * kGeoN : 1];
     const ddv3 ri = A.lddv(F_R, gi), vi = A.lddv(F_V, gi);
     const double sR2 = A.ld(F_SR2, gi), k2 = A.ld(F_K2, gi), td = A.ld(F_TD, gi);
     const bool damped = k2 != 0 && td != 0, ndelay = k2 != 0 && td == 0;
@@ -1343,7 +1347,22 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 1 ?
 #pragma unroll
             for (int q = 0; q < 13; ++q) t[q] = 0.0;
             if (jl < n && jl != il) {
-                if (!one_round) pair_geo(jl);
+                if (!one_round) {
+                    if (W == 4 && damped && n <= kGeoN && !first_pass) {
+                        pr = v3{geo[jl], geo[kGeoN + jl], geo[2 * kGeoN + jl]};
+                        prr2 = geo[3 * kGeoN + jl]; ptp_dphi = geo[4 * kGeoN + jl]; ptp_dg = geo[5 * kGeoN + jl];
+                        pv = v3{geo[6 * kGeoN + jl], geo[7 * kGeoN + jl], geo[8 * kGeoN + jl]};
+                        ptdf = geo[9 * kGeoN + jl];
+                    } else {
+                        pair_geo(jl);
+                        if (W == 4 && damped && n <= kGeoN) {
+                            geo[jl] = pr.x; geo[kGeoN + jl] = pr.y; geo[2 * kGeoN + jl] = pr.z;
+                            geo[3 * kGeoN + jl] = prr2; geo[4 * kGeoN + jl] = ptp_dphi; geo[5 * kGeoN + jl] = ptp_dg;
+                            geo[6 * kGeoN + jl] = pv.x; geo[7 * kGeoN + jl] = pv.y; geo[8 * kGeoN + jl] = pv.z;
+                            geo[9 * kGeoN + jl] = ptdf;
+                        }
+                    }
+                }
                 const v3 r = pr;
                 const double rr2 = prr2, tp_dphi = ptp_dphi, tp_dg = ptp_dg;
                 const v3 na = tp_dg * r;
@@ -1666,29 +1685,34 @@ __device__ __forceinline__ void stage_translational(const View &d, const Acc &A,
         A.stddv(F_X0R, g, A.lddv(F_R, g));
         A.stddv(F_X0V, g, v);
     }
+    const v3 a_now = (A.ldv(F_GACC, g) + A.ldv(F_DACC, g)) + A.ldv(F_NACC, g);
     {
         const int fb = F_STAGE + (k - 1) * kStageFields;
         A.stddv(fb, g, v);
-        A.stv(fb + 6, g, (A.ldv(F_GACC, g) + A.ldv(F_DACC, g)) + A.ldv(F_NACC, g));
+        A.stv(fb + 6, g, a_now);
     }
+    const ddv3 r0 = A.lddv(F_X0R, g), v0 = A.lddv(F_X0V, g);
+    const double gm0 = A.ld(F_GM0, g), dgm = A.ld(F_DGM, g), dj2 = A.ld(F_DJ2, g);
+    const double t = d.t0[d.bsys[g]] + tadd;
     ddv3 cv = mkddv(v3{0, 0, 0});
     v3 ca{0, 0, 0};
     const int base = k * (k - 1) / 2;
-    for (int j = 0; j < k; ++j) {
+    const int e1 = c_row0[k + 1];
+#pragma unroll 4
+    for (int e = c_row0[k]; e < e1; ++e) {              // non-zero columns, ascending
+        const int j = c_nzj[e];
         const double2 c = c_rk[base + j];
-        if (c.x == 0) continue;
         const dd ckj{c.x, c.y};
         const double fc = c.x + c.y;
         const int fb = F_STAGE + j * kStageFields;
-        cv = add(cv, scale(ckj, A.lddv(fb, g)));
-        ca += fc * A.ldv(fb + 6, g);
+        const bool now = j == k - 1;                    // this stage's record, still in registers
+        cv = add(cv, scale(ckj, now ? v : A.lddv(fb, g)));
+        ca += fc * (now ? a_now : A.ldv(fb + 6, g));
     }
-    const ddv3 r0 = A.lddv(F_X0R, g), v0 = A.lddv(F_X0V, g);
     A.stddv(F_R, g, add(r0, scale(mkdd(dt), cv)));
     A.stddv(F_V, g, add(v0, mkddv(dt * ca)));
-    const double t = d.t0[d.bsys[g]] + tadd;
-    A.st(F_GM, g, A.ld(F_GM0, g) + A.ld(F_DGM, g) * t);
-    A.st(F_EXJ2, g, A.ld(F_DJ2, g) * t);
+    A.st(F_GM, g, gm0 + dgm * t);
+    A.st(F_EXJ2, g, dj2 * t);
 }
 
 __device__ __forceinline__ void stage_rotational(const Acc &A, int g, int k, double dt, double dt_k) {
@@ -1720,20 +1744,23 @@ __device__ __forceinline__ void stage_rotational(const Acc &A, int g, int k, dou
         egrad = cross((t1 + cross(w, irtq)) + ir * ((cross(j, irtq) - 2 * cross(w, tq)) + tqdot), w);
         A.stv(F_EGRAD, g, egrad);
     }
-    v3 cw{0, 0, 0}, ct{0, 0, 0};
-    const int base = k * (k - 1) / 2;
-    for (int j = 0; j < k; ++j) {
-        const double2 c = c_rk[base + j];
-        if (c.x == 0) continue;
-        const double fc = c.x + c.y;
-        const int fb = F_STAGE + j * kStageFields;
-        cw += fc * A.ldv(fb + 9, g);
-        ct += fc * A.ldv(fb + 12, g);
-    }
     const ddv3 gl0 = A.lddv(F_X0GL, g);
     const v3 w0 = A.ldv(F_X0W, g);
-    const v3 w = cw / dt_k;
     ddm3 sm = A.lddm(F_X0S, g);
+    v3 cw{0, 0, 0}, ct{0, 0, 0};
+    const int base = k * (k - 1) / 2;
+    const int e1 = c_row0[k + 1];
+#pragma unroll 4
+    for (int e = c_row0[k]; e < e1; ++e) {              // non-zero columns, ascending
+        const int j = c_nzj[e];
+        const double2 c = c_rk[base + j];
+        const double fc = c.x + c.y;
+        const int fb = F_STAGE + j * kStageFields;
+        const bool now = j == k - 1;                    // this stage's record, still in registers
+        cw += fc * (now ? w_now : A.ldv(fb + 9, g));
+        ct += fc * (now ? tq_now : A.ldv(fb + 12, g));
+    }
+    const v3 w = cw / dt_k;
     const v3 dw = (2.0 / 3) * (w - w0);
     sm = add(sm, rotation_matrix(w - dw, dt * dt_k / 2, DevSin{}) * todbl(sm));
     sm = add(sm, rotation_matrix(w + dw, dt * dt_k / 2, DevSin{}) * todbl(sm));
