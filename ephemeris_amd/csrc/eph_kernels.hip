@@ -936,24 +936,27 @@ constexpr int kWpbEntMax = 64 * kWpbSlots + 16;      // entries of one tile, pad
 constexpr int kWpbTile = 3 * kWpbPitch + kWpbEnt * kWpbEntMax;
 
 // acc += row[0] + ... + row[63], one dependent addition at a time in index
-// order; the reads of the next eight values are issued before this eight
-// are added, so the chain does not wait for LDS.
+// order; the reads of the next sixteen values are issued before this
+// sixteen are added (the scheduling barriers keep the compiler from
+// gathering all reads ahead of one wait), so the chain does not wait for LDS.
 __device__ __forceinline__ void chain_add64(const double *row, double &acc) {
-    double b0[8], b1[8];
+    double b0[16], b1[16];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) b0[i] = row[i];
+    for (int i = 0; i < 16; ++i) b0[i] = row[i];
 #pragma unroll
-    for (int q = 0; q < 64; q += 16) {
+    for (int q = 0; q < 64; q += 32) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) b1[i] = row[q + 8 + i];
+        for (int i = 0; i < 16; ++i) b1[i] = row[q + 16 + i];
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int i = 0; i < 8; ++i) acc += b0[i];
-        if (q + 16 < 64) {
+        for (int i = 0; i < 16; ++i) acc += b0[i];
+        if (q + 32 < 64) {
 #pragma unroll
-            for (int i = 0; i < 8; ++i) b0[i] = row[q + 16 + i];
+            for (int i = 0; i < 16; ++i) b0[i] = row[q + 32 + i];
         }
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int i = 0; i < 8; ++i) acc += b1[i];
+        for (int i = 0; i < 16; ++i) acc += b1[i];
     }
 }
 // acc += p[0] + p[S] + ... + p[(cnt - 1) * S], one dependent addition at a
