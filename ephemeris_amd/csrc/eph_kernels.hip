@@ -1061,6 +1061,11 @@ __global__ __launch_bounds__(64 * W) void k_main_wpb(View d, const int2 *__restr
     v3 rfv{0, 0, 0}, tdA{0, 0, 0}, tqA{0, 0, 0}, tdB{0, 0, 0}, tqB{0, 0, 0};
     double ring8[8];
     int own0 = -1, own1 = -1;
+    // with four wavefronts, waves 2 and 3 also form the first own item's
+    // partner products during the last chains (partners lane + 128 m of their
+    // 128 lanes; a second own item, a ring, is formed after the loop)
+    constexpr bool kPre = W == 4;
+    double P0[4][6];
     for (int t0 = 0; t0 < n; t0 += 64 * W) {
         const int jl = t0 + 64 * wv + lane;
         unsigned keep = 0;
@@ -1116,13 +1121,35 @@ __global__ __launch_bounds__(64 * W) void k_main_wpb(View d, const int2 *__restr
             const int nw = min(W, (n - t0 + 63) / 64);
             for (int u = 0; u < nw; ++u) chain_add64(term + u * kWpbTile + lane * kWpbPitch, gacc);
         }
-        if (np && t0 + 64 * W >= n && wv == kHA) {
+        if (np && t0 + 64 * W >= n && (wv == kHA || (kPre && wv == kHB))) {
             int kind = -1;
-            if (lane < sm.nitem) rfv = np_recv_term(d, A, sm, il, gi, lane, kind);
+            if (lane < sm.nitem) {
+                const v3 f = np_recv_term(d, A, sm, il, gi, lane, kind);
+                if (wv == kHA) rfv = f;
+            }
             const uint64_t b0 = __ballot(kind == 0), b1 = __ballot(kind == 1);
             own0 = b0 ? __ffsll((unsigned long long)b0) - 1 : -1;
             own1 = b1 ? __ffsll((unsigned long long)b1) - 1 : -1;
-            if (lane == 0) np_finalize_pre(d, A, gi, I.phi, ring8);
+            if (wv == kHA && lane == 0) np_finalize_pre(d, A, gi, I.phi, ring8);
+            if (kPre && own0 >= 0) {
+                const int off0 = d.items[sm.item0 + own0].an_off;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const int jl = (wv - kHA) * 64 + lane + 128 * m;
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) P0[m][c] = 0;
+                    if (jl < n && jl != il) {
+                        const int gj = off + jl;
+                        const double GMj = A.ld(F_GM, gj);
+                        const v3 rij = diff(A.lddv(F_R, gj), I.r);
+                        const double *a = d.an + 3 * ((int64_t)off0 + jl);
+                        const v3 an{a[0], a[1], a[2]};
+                        const v3 ta = GMj * an, tt = GMj * cross(rij, an);
+                        P0[m][0] = ta.x; P0[m][1] = ta.y; P0[m][2] = ta.z;
+                        P0[m][3] = tt.x; P0[m][4] = tt.y; P0[m][5] = tt.z;
+                    }
+                }
+            }
         }
         __syncthreads();
     }
@@ -1130,6 +1157,18 @@ __global__ __launch_bounds__(64 * W) void k_main_wpb(View d, const int2 *__restr
     double *rf = term, *ringS = rf + 3 * kNpFusedMaxItems, *partA = ringS + 8,
            *partB = partA + 6 * kNpFusedMaxItems, *pq = term + kNpHead;
     constexpr bool kBoth = W * kWpbTile >= kNpHead + 12 * kNpFusedMaxN;   // both own items' products at once
+    static_assert(!kPre || kBoth, "the products formed during the chains use both items' regions");
+    if (kPre && np && (wv == kHA || wv == kHB) && own0 >= 0) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int jl = (wv - kHA) * 64 + lane + 128 * m;
+            if (jl < n) {
+                double *o = pq + 6 * jl;
+#pragma unroll
+                for (int c = 0; c < 6; ++c) o[c] = P0[m][c];
+            }
+        }
+    }
     if (np && wv == kHA) {
         if (lane < sm.nitem) { rf[3 * lane] = rfv.x; rf[3 * lane + 1] = rfv.y; rf[3 * lane + 2] = rfv.z; }
         if (lane == 0) {
@@ -1158,8 +1197,9 @@ __global__ __launch_bounds__(64 * W) void k_main_wpb(View d, const int2 *__restr
         const int q0 = own_s[0], q1 = own_s[1];
         for (int pass = 0; pass < (kBoth ? 1 : 2); ++pass) {
             const bool do0 = q0 >= 0 && (kBoth || pass == 0), do1 = q1 >= 0 && (kBoth || pass == 1);
-            if (do0 || do1) {
-                const int off0 = do0 ? d.items[sm.item0 + q0].an_off : 0;
+            const bool f0 = do0 && !kPre;          // item 0's products are formed already (kPre)
+            if (f0 || do1) {
+                const int off0 = f0 ? d.items[sm.item0 + q0].an_off : 0;
                 const int off1 = do1 ? d.items[sm.item0 + q1].an_off : 0;
                 double *p0 = pq, *p1 = pq + (kBoth ? 6 * kNpFusedMaxN : 0);
                 for (int jl = tid; jl < n; jl += 64 * W) {
@@ -1168,7 +1208,7 @@ __global__ __launch_bounds__(64 * W) void k_main_wpb(View d, const int2 *__restr
                         const int gj = off + jl;
                         const double GMj = A.ld(F_GM, gj);
                         const v3 rij = diff(A.lddv(F_R, gj), I.r);
-                        if (do0) {
+                        if (f0) {
                             const double *a = d.an + 3 * ((int64_t)off0 + jl);
                             const v3 an{a[0], a[1], a[2]};
                             a0 = GMj * an;
@@ -1181,7 +1221,7 @@ __global__ __launch_bounds__(64 * W) void k_main_wpb(View d, const int2 *__restr
                             t1v = GMj * cross(rij, an);
                         }
                     }
-                    if (do0) {
+                    if (f0) {
                         double *o = p0 + 6 * jl;
                         o[0] = a0.x; o[1] = a0.y; o[2] = a0.z; o[3] = t0v.x; o[4] = t0v.y; o[5] = t0v.z;
                     }
