@@ -88,3 +88,27 @@ def test_xcd_dealt_batch_matches_per_thread_variant(tmp_path_factory):
     b, tb = pt.download()
     bad = [f for f in a.dtype.names if not bits_equal(a[f], b[f])]
     assert not bad and bits_equal(ta, tb), bad
+
+
+def test_two_wavefront_fused_batch_matches_per_thread_variant(tmp_path_factory):
+    """A batch of six large systems on the one-workgroup-per-body kernels:
+    with more than 768 bodies the main force runs two wavefronts per body
+    with the non-point pass fused (the own items' products formed one item
+    at a time after the pair loop); bits equal the one-thread-per-body
+    kernels'."""
+    print("\n[case] 6-member batch of 300-504 bodies, variant 2 (two wavefronts, fused non-point) vs variant 1, 2 steps",
+          flush=True)
+    specs = gen.make_ensemble(6, seed=96, n_lo=300, n_hi=504)
+    cfgs = ensemble.write_members(specs, tmp_path_factory.mktemp("wpb2"))
+    masses, sys_n, gps, rings, t0 = ensemble.load_members(cfgs)
+    assert sys_n.sum() > 768 and len(gps) > 0 and len(rings) > 0
+    w2 = GpuSystems(masses, sys_n, gps, rings, t0)
+    pt = GpuSystems(masses, sys_n, gps, rings, t0)
+    w2.set_variant(2)
+    pt.set_variant(1)
+    w2.integrate(300.0, 2)
+    pt.integrate(300.0, 2)
+    a, ta = w2.download()
+    b, tb = pt.download()
+    bad = [f for f in a.dtype.names if not bits_equal(a[f], b[f])]
+    assert not bad and bits_equal(ta, tb), bad
