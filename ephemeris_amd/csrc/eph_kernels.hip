@@ -1051,6 +1051,7 @@ __global__ __launch_bounds__(64 * W) void k_main_wpb(View d, const int2 *__restr
     const IMain I = load_imain(A, gi);
     double mind0 = 0, maxi0 = 0;
     if (wv == 0) { mind0 = A.ld(F_MIND, gi); maxi0 = A.ld(F_MAXI, gi); }
+    const int ring_i = NP ? d.np_ring[gi] : -1;            // read early: the tail needs it last
     double *myg = term + wv * kWpbTile, *mye = myg + 3 * kWpbPitch;
     double acc = 0, gacc = 0;                              // lanes 0-5 of wave 0: daccel, dtorque; gaccel
     double mind = 0, maxi = 0;                             // lane-local maxima (>= +0: identities)
@@ -1294,7 +1295,7 @@ __global__ __launch_bounds__(64 * W) void k_main_wpb(View d, const int2 *__restr
             }
         }
         // (c) daccel and dtorque with RING_CORRECTION (phi and naccel are written)
-        if (d.np_ring[gi] < 0) {
+        if (ring_i < 0) {
             if (lane >= 3 && lane < 6) A.st(F_DACC + lane - 3, gi, acc);
             else if (lane >= 6 && lane < 9) A.st(F_DTQ + lane - 6, gi, acc);
         } else {
