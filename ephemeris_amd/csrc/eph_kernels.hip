@@ -831,6 +831,46 @@ __global__ __launch_bounds__(BS) void k_main_pool(View d, const int2 *__restrict
 // ---------------------------------------------------------------------------
 constexpr int kK4Pitch = 25;
 
+// pair_terms for the four-lane kernel, which only adds the terms to sums that
+// start at +0: a term that is exactly zero for finite inputs -- the own or
+// the partner's quadrupole when there is none, the partner-spin LT term of a
+// partner without spin, the radiation of a dark partner -- is left at +0
+// without being formed (adding a zero of either sign to such a sum is an
+// exact identity).  The branches are uniform whenever the wavefront's 16
+// bodies or 4 partners agree, which they mostly do.
+__device__ __forceinline__ PairTerms pair_terms_k4(const IMain &I, bool own_quad, const JMain &T) {
+    PairTerms o;
+    v3 unused{0, 0, 0};
+    const PairGeo p = relativity(I, T, unused);
+    o.G = p.G;
+    o.rr = p.rr;
+    o.tp_dg = p.tp_dg;
+    o.dg = v3{0, 0, 0};
+    o.rxdg = v3{0, 0, 0};
+    if (own_quad) {
+        o.dg = (T.GM * p.rr3 * p.rr2 * I.R2) * quad_field(I.C, p);
+        o.rxdg = cross(p.r, o.dg);
+    }
+    o.gj = v3{0, 0, 0};
+    bool cj = false;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) cj = cj || T.C[q] != 0;
+    if (cj && T.R2 != 0) {
+        const m3 Cj{v3{T.C[0], T.C[1], T.C[2]}, v3{T.C[3], T.C[4], T.C[5]}, v3{T.C[6], T.C[7], T.C[8]}};
+        o.gj = (T.GM * p.rr3 * p.rr2 * T.R2) * quad_field(Cj, p);
+    }
+    const double rcr32 = 2 * T.GM / kC * p.rr3;
+    const v3 GLb = cross(I.GL, p.b);
+    o.lt1 = rcr32 * (GLb - (3 * dot(GLb, p.r) * p.rr2) * p.r);
+    o.lt2 = rcr32 * cross(I.GL, cross(p.r, p.b));
+    const v3 GLj{T.GLx, T.GLy, T.GLz};
+    o.lt3 = v3{0, 0, 0};
+    if (GLj.x != 0 || GLj.y != 0 || GLj.z != 0) o.lt3 = cross(rcr32 * (GLj - (3 * dot(GLj, p.r) * p.rr2) * p.r), p.b);
+    o.rad = v3{0, 0, 0};
+    if (T.lum != 0) o.rad = (T.lum * I.rR2G4c * p.rr3 / I.GM) * p.r;
+    return o;
+}
+
 __global__ __launch_bounds__(64) void k_main_k4(View d, const int2 *__restrict__ blk) {
     const Acc A{d.F, d.nbp};
     const int s = blk[blockIdx.x].x, i0 = blk[blockIdx.x].y;   // bodies i0 .. i0 + 15
@@ -843,6 +883,13 @@ __global__ __launch_bounds__(64) void k_main_k4(View d, const int2 *__restrict__
     const int gi = off + (valid ? il : 0);
     __shared__ double term[64 * kK4Pitch];
     const IMain I = load_imain(A, gi);
+    bool own_quad = false;
+    {
+        const double *ci = &I.C.x.x;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) own_quad = own_quad || ci[q] != 0;
+        own_quad = own_quad && I.R2 != 0;
+    }
     const int c = k < 3 ? k : 0;
     double a_d = 0, a_g = 0, a_q = 0;                      // daccel, gaccel, dtorque component c
     double mind = 0, maxi = 0;
@@ -853,7 +900,7 @@ __global__ __launch_bounds__(64) void k_main_k4(View d, const int2 *__restrict__
         if (valid && jl < n && jl != il) {
             JMain T;
             load_jmain(A, T, off + jl);
-            const PairTerms pt = pair_terms(I, T);
+            const PairTerms pt = pair_terms_k4(I, own_quad, T);
             const double v[24] = {pt.G.x, pt.G.y, pt.G.z, pt.dg.x, pt.dg.y, pt.dg.z, pt.gj.x, pt.gj.y, pt.gj.z,
                                   -pt.lt1.x, -pt.lt1.y, -pt.lt1.z, pt.lt3.x, pt.lt3.y, pt.lt3.z,
                                   pt.rad.x, pt.rad.y, pt.rad.z, pt.rxdg.x, pt.rxdg.y, pt.rxdg.z,
