@@ -229,11 +229,16 @@ int Server::admit(int kind, std::vector<SJob *> &fresh) {
         const bool mem_ok = mem_used + j->bytes <= mem_budget &&
                             (int64_t)free_b >= reserve + resident + 2 * j->bytes;
         if (!slots_ok || !mem_ok) {
-            if (n_run[kind] == 0 && fresh.empty()) {     // could never be admitted
+            // Failed for good only when it could not be admitted even with no
+            // job of either kind resident (the budget was checked at
+            // submission); otherwise it waits for jobs to finish.
+            const bool never_slots = max_slots && want > max_slots;
+            const bool never_mem = n_running == 0 && fresh.empty() && (int64_t)free_b < 2 * j->bytes;
+            if (never_slots || never_mem) {
                 qi = queue.erase(qi);
                 --n_queued[kind];
                 j->status = EPH_JOB_FAILED;
-                j->error = slots_ok ? "job does not fit in free device memory" : "job exceeds the server's slots";
+                j->error = never_slots ? "job exceeds the server's slots" : "job does not fit in free device memory";
                 j->t_done = clock_s(t0);
                 ++st.jobs_failed;
                 continue;

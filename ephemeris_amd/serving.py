@@ -135,6 +135,10 @@ def latency_summary(recs):
     if len(lat):
         out.update(p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
                    mean_s=float(lat.mean()), max_s=float(lat.max()))
+        if len(lat) < 100:
+            # with fewer than 100 samples the 99th percentile is interpolated
+            # between the two slowest jobs, so it is close to the maximum
+            out["p99_note"] = f"{len(lat)} jobs: p99 lies between the two slowest latencies"
     for kind in ("particles", "ensemble"):
         k = np.array([r["latency"] for r in recs if r["status"] == "done" and r["kind"] == kind])
         if len(k):

@@ -6,16 +6,18 @@ replays the same call sequence, because the epoch used for the stage times
 is taken at the start of each call.  Bar: max relative position difference
 <= 1e-12 at every checkpoint (1 simulated year is 105,192 steps).
 
-The 10-day checkpoint runs in the default GPU suite; the full year (about
-five minutes on one MI355X) runs with EPH_LONG=1.
+One GPU system is carried through the checkpoints in order, one test per
+checkpoint, so the year ends in the last test (about two minutes of GPU
+time after the 100-day checkpoint).  A test run on its own first replays the
+earlier intervals.  Beside the position and velocity differences each
+checkpoint reports the drift of linear and angular momentum (with spin) of
+the GPU trajectory against the oracle's, both relative to the epoch value.
 """
-import os
-
 import numpy as np
 import pytest
 
 from ephemeris_amd.system import GpuSystems
-from tests.util import GOLDEN, load_npz, position, velocity
+from tests.util import GOLDEN, load_npz, momenta, position, velocity
 
 pytestmark = pytest.mark.gpu
 
@@ -23,44 +25,44 @@ CHECKPOINTS = [288, 2880, 8640, 28800, 105192]
 POS_TOL = 1e-12
 
 
-def _run(upto):
-    d = load_npz(GOLDEN / "accel_full504_pool.npz")
-    g = GpuSystems(d["masses_in"], None, d["gps"], d["rings"])
+class _Carry:
+    """The GPU system and the number of steps it has taken."""
+    g = None
     done = 0
-    rows = []
-    for ck in CHECKPOINTS:
-        if ck > upto:
-            break
-        g.integrate(300.0, ck - done)
-        done = ck
-        path = GOLDEN / f"traj_full504_{ck}.npz"
-        if not path.exists():
-            continue
-        ref = load_npz(path)
-        m, t = g.download()
-        r, rr = position(m), position(ref)
-        dr = np.max(np.linalg.norm(r - rr, axis=1) / np.linalg.norm(rr, axis=1))
-        dv = np.max(np.linalg.norm(velocity(m) - velocity(ref), axis=1) /
-                    np.maximum(np.linalg.norm(velocity(ref), axis=1), 1e-30))
-        da = np.max(np.abs(r - rr))
-        print(f"full504 after {ck} steps ({ck / 288:.2f} days): max rel dr={dr:.3e} dv={dv:.3e} "
-              f"max |dr|={da:.3e} m, t_eph equal={t[0].sum() == float(ref['t_eph'])}", flush=True)
-        rows.append((ck, dr))
-        assert t[0].sum() == float(ref["t_eph"])
-    return rows
+    p0 = None
 
 
-def test_full504_ten_days():
-    print("\n[case] full504 to 10 days (288 + 2592 steps)", flush=True)
-    rows = _run(2880)
-    assert rows and all(dr <= POS_TOL for _, dr in rows), rows
+def _advance_to(ck):
+    if _Carry.g is None or _Carry.done > ck:
+        d = load_npz(GOLDEN / "accel_full504_pool.npz")
+        _Carry.g = GpuSystems(d["masses_in"], None, d["gps"], d["rings"])
+        _Carry.done = 0
+        _Carry.p0 = momenta(d["masses_in"])
+    for c in CHECKPOINTS:
+        if _Carry.done < c <= ck:
+            _Carry.g.integrate(300.0, c - _Carry.done)
+            _Carry.done = c
+    return _Carry.g
 
 
-@pytest.mark.slow
-@pytest.mark.timeout(1200)
-@pytest.mark.skipif(os.environ.get("EPH_LONG") != "1", reason="one simulated year takes minutes; set EPH_LONG=1")
-def test_full504_one_year():
-    upto = int(os.environ.get("EPH_LONG_STEPS", "105192"))
-    print(f"\n[case] full504 to {upto} steps (one year = 105192)", flush=True)
-    rows = _run(upto)
-    assert rows and all(dr <= POS_TOL for _, dr in rows), rows
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("ck", CHECKPOINTS)
+def test_full504_checkpoint(ck):
+    print(f"\n[case] full504 to {ck} steps ({ck / 288:.2f} days)", flush=True)
+    g = _advance_to(ck)
+    ref = load_npz(GOLDEN / f"traj_full504_{ck}.npz")
+    m, t = g.download()
+    r, rr = position(m), position(ref)
+    dr = np.max(np.linalg.norm(r - rr, axis=1) / np.linalg.norm(rr, axis=1))
+    dv = np.max(np.linalg.norm(velocity(m) - velocity(ref), axis=1) /
+                np.maximum(np.linalg.norm(velocity(ref), axis=1), 1e-30))
+    da = np.max(np.abs(r - rr))
+    p0 = _Carry.p0
+    pg, po = momenta(m), momenta(ref)
+    drift = {k: (np.linalg.norm(pg[k] - p0[k]) / np.linalg.norm(p0[k]),
+                 np.linalg.norm(po[k] - p0[k]) / np.linalg.norm(p0[k])) for k in p0}
+    print(f"full504 after {ck} steps: max rel dr={dr:.3e} dv={dv:.3e} max |dr|={da:.3e} m; "
+          + "; ".join(f"{k} drift GPU {a:.3e} oracle {b:.3e}" for k, (a, b) in drift.items())
+          + f"; t_eph equal={t[0].sum() == float(ref['t_eph'])}", flush=True)
+    assert t[0].sum() == float(ref["t_eph"])
+    assert dr <= POS_TOL, dr

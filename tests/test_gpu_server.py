@@ -132,3 +132,34 @@ def test_admission_by_memory_budget_and_worker(setup):
     with pytest.raises(Exception):
         srv.submit_ensemble(*one, n_steps=7)                        # not a multiple of the quantum
     srv.close()
+
+
+def test_mixed_kinds_wait_under_a_tight_budget(setup):
+    """A job that fits the budget alone waits while a job of the other kind
+    holds part of it, and runs once that job is done."""
+    cfg, members, r, v = setup
+    print("\n[case] server budget: ensemble job queued behind a particle job", flush=True)
+    one = _job(members, [0])
+    probe = Server(_field(cfg), ens_dt=ENS_DT, ens_quantum=ENS_Q, part_dt=PART_DT, part_quantum=PART_Q,
+                   part_max_steps=PART_MAX)
+    pb = probe.info(probe.submit_particles(r[:600], v[:600], PART_MAX))["bytes"]
+    eb = probe.info(probe.submit_ensemble(*one, n_steps=6))["bytes"]
+    probe.close()
+    budget = max(pb, eb) + min(pb, eb) // 2                  # each alone fits, both together do not
+    srv = Server(_field(cfg), ens_dt=ENS_DT, ens_quantum=ENS_Q, part_dt=PART_DT, part_quantum=PART_Q,
+                 part_max_steps=PART_MAX, mem_budget=int(budget))
+    pj = srv.submit_particles(r[:600], v[:600], PART_MAX)
+    srv.tick()
+    assert srv.info(pj)["status"] not in (FAILED,)
+    ej = srv.submit_ensemble(*one, n_steps=6)
+    srv.tick()
+    st_e = srv.info(ej)["status"]
+    print(f"budget {budget} B (particle job {pb} B, ensemble job {eb} B); ensemble status after one tick: {st_e}",
+          flush=True)
+    assert st_e != FAILED
+    srv.run_until_idle()
+    assert srv.info(pj)["status"] == DONE and srv.info(ej)["status"] == DONE
+    m, t = srv.result_ensemble(ej)
+    ma, ta = _alone_ensemble(one, 6)
+    assert bits_equal(m, ma) and bits_equal(t, ta)
+    srv.close()

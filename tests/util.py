@@ -34,3 +34,27 @@ def position(m):
 
 def velocity(m):
     return np.stack([dd_value(m, "v", c) for c in "xyz"], -1)
+
+
+def spin(m):
+    return np.stack([dd_value(m, "GL", c) for c in "xyz"], -1)
+
+
+def momenta(m, rings=None):
+    """Linear and angular momentum (per G) of a system, as the reference's
+    conservation test forms them (src/tests/test_conservation.cpp:55-80):
+    sum GM v and sum GM (r x v + GL); with `rings`, a ring host also carries
+    its ring's share of GM and the ring's own angular momentum."""
+    gm = m["GM"][:, None]
+    r, v, gl = position(m), velocity(m), spin(m)
+    p = gm * v
+    a = gm * np.cross(r, v)
+    l = a + gm * gl
+    if rings is not None:
+        for i in np.nonzero(m["ring"] >= 0)[0]:
+            rg = rings[m["ring"][i]]
+            f = rg["GM_ratio"] / (1 - rg["GM_ratio"])
+            p[i] = p[i] + p[i] * f
+            u = gl[i] / np.linalg.norm(gl[i])
+            l[i] = l[i] + a[i] * f + u * (rg["GL_R2"] * m["GM"][i] * m["R2"][i])
+    return {"linear": p.sum(0), "angular": l.sum(0)}
