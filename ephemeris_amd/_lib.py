@@ -58,13 +58,29 @@ SIGNATURES = {
     "eph_server_stats_read": (I, [P, P]),
     "eph_server_destroy": (I, [P]),
     "eph_msystem_load": (I, [C.c_char_p, I, PP]),
+    "eph_msystem_create": (I, [I64, P, P, I64, P, I64, P, PP]),
+    "eph_msystem_load_checkpoint": (I, [C.c_char_p, PP]),
+    "eph_msystem_save_checkpoint": (I, [P, C.c_char_p]),
     "eph_msystem_destroy": (I, [P]),
     "eph_msystem_size": (I64, [P]),
     "eph_msystem_masses": (I, [P, P, I64]),
+    "eph_msystem_set_masses": (I, [P, P, I64]),
     "eph_msystem_models": (I, [P, P, I64, P, I64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "eph_msystem_time": (I, [P, C.POINTER(C.c_double)]),
+    "eph_msystem_set_time": (I, [P, C.POINTER(C.c_double)]),
+    "eph_msystem_params": (I, [P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "eph_msystem_accel": (I, [P, I]),
     "eph_msystem_integrate": (I, [P, D, I64, I]),
+    "eph_msystem_combined_integrate": (I, [P, D, I64, I64, I]),
+    "eph_msystem_update": (I, [P, D]),
+    "eph_msystem_analyse": (I, [P, I, C.POINTER(C.c_double)]),
+    "eph_msystem_barycens": (I64, [P, P, P, P, I64, I64]),
+    "eph_msystem_scale": (I, [P, I64, D]),
+    "eph_msystem_scale_geopotential": (I, [P, I64, D]),
+    "eph_msystem_scale_ring": (I, [P, I64, D]),
+    "eph_msystem_set_tidal": (I, [P, I64, P, P, I64]),
+    "eph_msystem_set_threads": (I, [P, I]),
+    "eph_msystem_get_mid": (I64, [P, C.c_char_p]),
     "eph_msystem_diagnostics": (I, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_double),
                                     C.POINTER(C.c_double), I64]),
 }

@@ -57,180 +57,18 @@ struct Acc {
 };
 
 // ---------------------------------------------------------------------------
-// geopotential::sum (src/physics/geopotential.impl:64-144)
+// ring::sum (src/physics/ring.impl:2070-2196) on the device tables
 // ---------------------------------------------------------------------------
-__device__ v3 gp_sum(const eph_geopotential &gp, double R, v3 r) {
-    double rho2 = r.x * r.x + r.y * r.y;
-    double rr2 = 1 / (rho2 + r.z * r.z);
-    double rho1 = sqrt(rho2);
-    double rr1 = sqrt(rr2);
-    double rb1 = R * rr1;
-    double rbn = rb1;
-    double cphi = rho1 * rr1;
-    double sphi = r.z * rr1;
-    double cm, sm;
-    if (rho2 < 2.2250738585072014e-308) { cm = 1; sm = 0; }
-    else { cm = r.x / rho1; sm = r.y / rho1; }
-    // Both loops are unrolled to the maximum degree (the loader rejects
-    // larger ones), so every index of pnm / pn1m is a constant and the
-    // arrays live in registers, not in scratch memory.
-    double pnm[EPH_GP_MAX_DEGREE], pn1m[EPH_GP_MAX_DEGREE];
-    pnm[0] = 0;
-    pn1m[0] = 1;
-    double ppn = 1, pn = 1, pn1 = sphi;
-    double zx = 0, zz = 0;
-    v3 t{0, 0, 0};
-    const int Nz = (int)gp.Nz, Nt = (int)gp.Nt;
-#pragma unroll
-    for (int n = 2; n <= EPH_GP_MAX_DEGREE; ++n) {
-        if (n > Nz) break;
-        rbn *= rb1;
-        const double *JCSn = gp.c + (n * n - 4);
-        pn = ((double)(2 * n - 1) * sphi * pn1 - (double)(n - 1) * pn) / (double)n;
-        ppn = sphi * ppn + (double)n * pn1;
-        double z1 = JCSn[0] * rbn;
-        zx += (double)(n + 1) * z1 * pn;
-        zz -= z1 * ppn;
-        { double sw = pn; pn = pn1; pn1 = sw; }
-        if (n > Nt) continue;
-        v3 f{0, 0, 0};
-        double m1 = (double)(2 * n - 1) * sphi;
-        double m2 = (double)(-n) * sphi;
-        double smm = 0, cmm = 0, smm1 = 0, cmm1 = 0;
-#pragma unroll
-        for (int m = 1; m <= EPH_GP_MAX_DEGREE; ++m) {
-            if (m > n) break;
-            if (m == n) {
-                pn1m[m - 1] = 0;
-                pnm[m - 1] = (double)(2 * n - 1) * cphi * pnm[m - 2];
-            } else
-                pnm[m - 1] = (m1 * pn1m[m - 1] - (double)(n + m - 1) * pnm[m - 1]) / (double)(n - m);
-            if (m == 1) { smm = sm; cmm = cm; smm1 = 0; cmm1 = 1; }
-            else { smm = 2 * cm * smm1 - smm; cmm = 2 * cm * cmm1 - cmm; }
-            double t1 = JCSn[2 * m] * smm + JCSn[2 * m - 1] * cmm;
-            double t2 = JCSn[2 * m] * cmm - JCSn[2 * m - 1] * smm;
-            double ppnm = m2 * pnm[m - 1] + (double)(n + m) * pn1m[m - 1];
-            f.x += t1 * pnm[m - 1];
-            f.y += (double)m * t2 * pnm[m - 1];
-            f.z += t1 * ppnm;
-            { double sw = cmm; cmm = cmm1; cmm1 = sw; }
-            { double sw = smm; smm = smm1; smm1 = sw; }
-            { double sw = pnm[m - 1]; pnm[m - 1] = pn1m[m - 1]; pn1m[m - 1] = sw; }
-        }
-        f.x *= (double)(n + 1);
-        t += rbn * f;
-    }
-    t.x = zx - cphi * t.x;
-    t.z += zz * cphi;
-    t = rr2 * t;
-    double g1 = t.x * cphi - t.z * sphi;
-    return v3{g1 * cm - t.y * sm, g1 * sm + t.y * cm, cphi * t.z + sphi * t.x};
-}
-
-// ---------------------------------------------------------------------------
-// ring::sum (src/physics/ring.impl:2070-2196)
-// ---------------------------------------------------------------------------
-__device__ float fzcorr(float x, float y, int kh) {
-    const int N = 32;
-    x *= (float)N;
-    y *= (float)N;
-    // cell indices clamped to the table while still floats, so a non-finite
-    // argument (a system gone bad) still reads inside the table; for finite
-    // arguments this is the integer clamp of ring.impl:2072-2079
-    const float fi = fminf(fmaxf(floorf(x), 0.0f), (float)(N - 1));
-    const float fj = fminf(fmaxf(floorf(y) - (float)(kh * N), 0.0f), (float)(N - 1));
-    const int i = (int)fi, j = (int)fj + kh * N;
-    x -= (float)i;
-    y -= (float)j;
-    const float *p = g_fz + 16 * (2 * N * i + j);
-    return p[0] + x * (p[4] + x * (p[8] + x * p[12]))
-         + y * (p[1] + x * (p[5] + x * (p[9] + x * p[13]))
-         + y * (p[2] + x * (p[6] + x * (p[10] + x * p[14]))
-         + y * (p[3] + x * (p[7] + x * (p[11] + x * p[15])))));
-}
-template <int NC>
-__device__ double padesum(double x, const double (&coef)[NC][2]) {
-    double resn = 0, resd = 0;
-    int n = NC;
-    while (n > 0) {
-        --n;
-        resn = x * resn + coef[n][0];
-        resd = x * resd + coef[n][1];
-    }
-    return resn / resd;
-}
-
-__device__ v3 ring_sum(const eph_ring &rg, double host_R, v3 r) {
-    const double pi = kPi;
-    bool psign = r.z < 0;
-    if (psign) r.z = -r.z;
-    const double re2 = r.x * r.x + r.y * r.y;
-    const double re = sqrt(re2);
-    const double rz2 = r.z * r.z;
-    const double r2 = re2 + rz2;
-    const double rz = r.z;
-    v3 ret{0, 0, 0};
-    const int N = (int)rg.N;
-    for (int i = 0; i < N; ++i) {
-        double R = rg.R_R[i] * host_R;
-        double H = rg.H_R[i] * host_R;
-        double reR = re * R;
-        double R2 = R * R, H2, fzh = 1;
-        double k2dre = 4 / (r2 + R2 + 2 * reR);
-        if (H > 0) {
-            H2 = H * H;
-            k2dre -= H2 * (k2dre * k2dre) * (1 - k2dre * rz2) / 48;
-            fzh = (re > R ? re - R : 0);
-            fzh = 4 * (rz2 + fzh * fzh);
-            fzh = sqrt(fzh / (fzh + H2));
-        }
-        double k2 = reR * k2dre;
-        k2dre *= R2 * sqrt(k2dre);
-        double phi = atan2(rz, re - R);
-        double k = 0, lk = 0, fx;
-        bool lkh = k2 >= 0.45;
-        if (lkh) {
-            k = sqrt(k2);
-            lk = log((1 - k2) / (8 * (1 + k)));
-            fx = (padesum(k, c_xserkk) + lk * padesum(k, c_xserkm)) / (k2 * k2);
-        } else {
-            fx = padesum(k2, c_xser0);
-        }
-        fx *= k2dre;
-        double fz;
-        double s = sin(phi), c = cos(phi);
-        bool kh = k2 >= .5;
-        bool exphi;
-        if (kh) {
-            const double phimax = (pi / 2);
-            exphi = c > 0;
-            double ck = 1 - k2, lk2 = log(ck / 16), sk = sqrt(ck);
-            double cc = exphi ? -c : c;
-            double cphi = exphi ? pi - phi : phi;
-            fz = cphi - (s / 8) * ((lk2 * (k2 - 5) - 2 * ck) * sk + 2 * cc * (1 + lk2) * ck);
-            fz *= fzcorr((float)((exphi ? phi : pi - phi) * (1 / phimax)), (float)(k2 * 2), 1);
-        } else {
-            const double phimax = (809. / 512);
-            exphi = phi > phimax;
-            double cr2 = exphi ? R2 + rz2 : r2;
-            double cRdr2 = (exphi ? re2 : R2) / cr2;
-            fz = (pi / 8) * (4 - 3 * cRdr2) * (rz / sqrt(cr2) * cRdr2);
-            fz *= fzcorr((float)((exphi ? pi - phi : phi) * (1 / phimax)), (float)(k2 * 2), 0);
-        }
-        if (exphi) {
-            double ek;
-            if (lkh) ek = padesum(k, c_kserkk) + lk * padesum(k, c_kserkm);
-            else ek = padesum(k2, c_kser0);
-            fz = pi - 2 * sqrt(1 - k2) * ek * s - fz;
-        }
-        double Gs = rg.GsR2[i] / (host_R * host_R);
-        fx *= Gs;
-        fz *= Gs;
-        ret += v3{-4 * fx * r.x, -4 * fx * r.y, -2 * fz * fzh};
-    }
-    if (psign) ret.z = -ret.z;
-    return ret;
-}
+struct DevRingTab {
+    __device__ static float fz(float x, float y, int kh) { return fzcorr_tab(g_fz, x, y, kh); }
+    __device__ static double xserkk(double x) { return padesum(x, c_xserkk); }
+    __device__ static double xserkm(double x) { return padesum(x, c_xserkm); }
+    __device__ static double xser0(double x) { return padesum(x, c_xser0); }
+    __device__ static double kserkk(double x) { return padesum(x, c_kserkk); }
+    __device__ static double kserkm(double x) { return padesum(x, c_kserkm); }
+    __device__ static double kser0(double x) { return padesum(x, c_kser0); }
+};
+__device__ v3 ring_sum(const eph_ring &rg, double host_R, v3 r) { return ring_sum_t<DevRingTab>(rg, host_R, r); }
 
 // world-frame acceleration (per unit GM of the non-point body p) at offset r
 // from p: geopotential (mass.cpp:186-188) or ring (mass.cpp:200-203)
@@ -261,16 +99,7 @@ __device__ void finalize_with(const View &d, const Acc &A, int g, v3 ga, v3 da, 
     const v3 nacc = A.ldv(F_NACCC, g) * kC2;
     A.stv(F_NACC, g, nacc);
     const int rg = d.np_ring[g];
-    if (rg >= 0) {
-        const eph_ring &mr = d.rings[rg];
-        da = da - mr.GM_ratio * ((ga + da) + nacc);
-        v3 mGL = todbl(A.lddv(F_GL, g));
-        const double rmGL = normalize(mGL);
-        const v3 ptorque = dot(dq, mGL) * mGL;
-        v3 dtq = dq - ptorque;
-        dtq = (rmGL / (rmGL + mr.GL_R2 * A.ld(F_R2, g))) * dtq;
-        dq = dtq + ptorque;
-    }
+    if (rg >= 0) ring_correction(d.rings[rg], A.ld(F_R2, g), todbl(A.lddv(F_GL, g)), ga, nacc, da, dq);
     A.stv(F_DACC, g, da);
     A.stv(F_DTQ, g, dq);
 }
@@ -349,44 +178,20 @@ struct PartnerBlock {
 // ---------------------------------------------------------------------------
 struct JDeform { double rxh, rxl, ryh, ryl, rzh, rzl, vxh, vxl, vyh, vyl, vzh, vzl, GM, sR2, tdf, pad; };
 
-// Per-body constants of UPDATE_HARMONICS (mass.impl:48-61); the static part
-// s C_static(t) s^T does not depend on the spin, so it is formed once.
-struct Harmonics {
-    double k2r, scale, R2, A;
-    m3 Cw;
-    __device__ void load(const Acc &Ac, int g) {
-        k2r = Ac.ld(F_K2R, g);
-        R2 = Ac.ld(F_R2, g);
-        A = Ac.ld(F_A, g);
-        scale = Ac.ld(F_RAD, g) * R2 / (2 * Ac.ld(F_GM, g));
-        const double exJ2 = Ac.ld(F_EXJ2, g);
-        m3 mc = Ac.ldm(F_CST, g);
-        mc.x.x += exJ2 / 2;
-        mc.y.y += exJ2 / 2;
-        mc.z.z -= exJ2;
-        Cw = toworld(todbl(Ac.lddm(F_S, g)), mc);
-    }
-    // C_potential and GI from the tidal sum and the spin w
-    __device__ void apply(const m3 &Csum, v3 w, m3 &C, m3 &GI) const {
-        const double w2 = dot(w, w);
-        C = adddiag(Csum, w2 * k2r / 3);
-        C = C - outer(k2r * w, w);
-        C = scale * C;
-        C = C + Cw;
-        GI = (-2.0 / 3 * R2) * subdiag(C, A);
-    }
-};
+// Per-body constants of UPDATE_HARMONICS (mass.impl:48-61) from the state
+__device__ __forceinline__ Harmonics harmonics_load(const Acc &Ac, int g) {
+    Harmonics h;
+    h.set(Ac.ld(F_K2R, g), Ac.ld(F_RAD, g), Ac.ld(F_R2, g), Ac.ld(F_A, g), Ac.ld(F_GM, g), Ac.ld(F_EXJ2, g),
+          Ac.ldm(F_CST, g), todbl(Ac.lddm(F_S, g)));
+    return h;
+}
 
 // UPDATE_ANGULAR_VELOCITY (mass.impl:63-72); returns should_break
 __device__ __forceinline__ bool update_spin(const Acc &A, int g, const m3 &GI, v3 &w, const ddv3 &GL) {
-    if (finite(GL)) {
-        const v3 oldw = w;
-        w = inverse(GI) * todbl(GL);
-        const v3 dwv = w - oldw;
-        return dot(dwv, dwv) < 1e-18 * dot(oldw, oldw);
-    }
-    A.stddv(F_GL, g, mkddv(GI * w));
-    return true;
+    bool gl_from_w;
+    const bool brk = spin_step(GI, w, GL, gl_from_w);
+    if (gl_from_w) A.stddv(F_GL, g, mkddv(GI * w));
+    return brk;
 }
 
 __device__ __forceinline__ void load_jdeform(const Acc &A, JDeform &T, int gj) {
@@ -462,7 +267,7 @@ __global__ __launch_bounds__(BS) void k_deform_light(View d, const int4 *__restr
     }
     if (!valid) return;
     Harmonics H;
-    H.load(A, gi);
+    H = harmonics_load(A, gi);
     v3 w = A.ldv(F_W, gi);
     const ddv3 GL = A.lddv(F_GL, gi);
     m3 C, GI;
@@ -572,7 +377,7 @@ __global__ __launch_bounds__(64) void k_deform_k4(View d, const int4 *__restrict
     const v3 nacc{v[0][0], v[0][1], v[0][2]};
     const m3 Csum{v3{acc1, v[1][0], v[1][1]}, v3{v[1][2], acc2, v[2][0]}, v3{v[2][1], v[2][2], acc3}};
     Harmonics H;
-    H.load(A, gi);
+    H = harmonics_load(A, gi);
     v3 w = A.ldv(F_W, gi);
     const ddv3 GL = A.lddv(F_GL, gi);
     m3 C, GI;
@@ -601,7 +406,7 @@ __global__ __launch_bounds__(BS) void k_deform_damped(View d, const int4 *__rest
     const double sR2 = A.ld(F_SR2, gi), k2 = A.ld(F_K2, gi), td = A.ld(F_TD, gi);
     const ddv3 GL = A.lddv(F_GL, gi);
     Harmonics H;
-    H.load(A, gi);
+    H = harmonics_load(A, gi);
     v3 w = A.ldv(F_W, gi);
     double phi = 0;
     v3 nacc{0, 0, 0};
@@ -678,13 +483,6 @@ __device__ __forceinline__ void load_jmain(const Acc &A, JMain &T, int gj) {
     T.lum = A.ld(F_LUM, gj);
 }
 
-// own-body data of the main force
-struct IMain {
-    ddv3 r;
-    double sR2, phi, beta2, R2, GM, rR2G4c;
-    v3 beta, GL;
-    m3 C;
-};
 __device__ __forceinline__ IMain load_imain(const Acc &A, int gi) {
     IMain I;
     I.r = A.lddv(F_R, gi);
@@ -695,89 +493,6 @@ __device__ __forceinline__ IMain load_imain(const Acc &A, int gi) {
     I.C = A.ldm(F_CPOT, gi);
     return I;
 }
-
-struct PairGeo { v3 r, b, G; double rr2, rr, rr3, tp_dg; };
-
-// RELATIVITY (mass.impl:82-97)
-__device__ __forceinline__ PairGeo relativity(const IMain &I, const JMain &T, v3 &gaccel) {
-    PairGeo p;
-    const ddv3 rj{dd{T.rxh, T.rxl}, dd{T.ryh, T.ryl}, dd{T.rzh, T.rzl}};
-    p.r = diff(rj, I.r);
-    p.rr2 = 1 / smax(dot(p.r, p.r), I.sR2 + T.sR2);
-    p.rr = sqrt(p.rr2);
-    p.rr3 = p.rr * p.rr2;
-    const double tp_dphi = p.rr * T.GM;
-    p.tp_dg = p.rr2 * tp_dphi;
-    const v3 bj{T.bx, T.by, T.bz}, nj{T.nx, T.ny, T.nz};
-    const double rbj = dot(p.r, bj);
-    double tp_rbjrr2 = rbj * p.rr;
-    tp_rbjrr2 *= tp_rbjrr2;
-    const double delta1 = 4 * I.phi + T.phi + I.beta2 + 2 * T.beta2
-                        - 4 * dot(I.beta, bj) + (dot(p.r, nj) - 3 * tp_rbjrr2) / 2;
-    p.b = bj - I.beta;
-    p.G = ((7 * tp_dphi / 2) * nj + (p.tp_dg * delta1) * p.r) + (p.tp_dg * (rbj - dot(p.r, p.b) * 4)) * p.b;
-    gaccel += p.G;
-    return p;
-}
-
-__device__ __forceinline__ v3 quad_field(const m3 &C, const PairGeo &p) {
-    const v3 Cr = C * p.r;
-    return (dot(p.r, Cr) * 5 * p.rr2) * p.r - (Cr + Cr);
-}
-
-// LENSE_THIRRING (mass.impl:99-106) and RADIATION_PRESSURE (:108-109)
-__device__ __forceinline__ void lt_rad(const IMain &I, const JMain &T, const PairGeo &p, v3 &da, v3 &dq) {
-    const double rcr32 = 2 * T.GM / kC * p.rr3;
-    const v3 GLb = cross(I.GL, p.b);
-    da -= rcr32 * (GLb - (3 * dot(GLb, p.r) * p.rr2) * p.r);
-    dq -= rcr32 * cross(I.GL, cross(p.r, p.b));
-    const v3 GLj{T.GLx, T.GLy, T.GLz};
-    da += cross(rcr32 * (GLj - (3 * dot(GLj, p.r) * p.rr2) * p.r), p.b);
-    da += (T.lum * I.rR2G4c * p.rr3 / I.GM) * p.r;
-}
-
-// Every term one pair (i, j) adds to i in the thread-pool order
-// (RELATIVITY, ROTATIONAL_TIDAL_DEFORMATION_NANTI_FORCE, LENSE_THIRRING,
-// RADIATION_PRESSURE; mass.impl:82-125, mass.cpp:147-155).
-struct PairTerms { v3 G, dg, rxdg, gj, lt1, lt2, lt3, rad; double rr, tp_dg; };
-__device__ __forceinline__ PairTerms pair_terms(const IMain &I, const JMain &T) {
-    PairTerms o;
-    v3 unused{0, 0, 0};
-    const PairGeo p = relativity(I, T, unused);
-    o.G = p.G;
-    o.rr = p.rr;
-    o.tp_dg = p.tp_dg;
-    o.dg = (T.GM * p.rr3 * p.rr2 * I.R2) * quad_field(I.C, p);
-    o.rxdg = cross(p.r, o.dg);
-    const m3 Cj{v3{T.C[0], T.C[1], T.C[2]}, v3{T.C[3], T.C[4], T.C[5]}, v3{T.C[6], T.C[7], T.C[8]}};
-    o.gj = (T.GM * p.rr3 * p.rr2 * T.R2) * quad_field(Cj, p);
-    const double rcr32 = 2 * T.GM / kC * p.rr3;
-    const v3 GLb = cross(I.GL, p.b);
-    o.lt1 = rcr32 * (GLb - (3 * dot(GLb, p.r) * p.rr2) * p.r);
-    o.lt2 = rcr32 * cross(I.GL, cross(p.r, p.b));
-    const v3 GLj{T.GLx, T.GLy, T.GLz};
-    o.lt3 = cross(rcr32 * (GLj - (3 * dot(GLj, p.r) * p.rr2) * p.r), p.b);
-    o.rad = (T.lum * I.rR2G4c * p.rr3 / I.GM) * p.r;
-    return o;
-}
-
-// Add one pair's terms in the reference order.
-struct MainAcc {
-    v3 ga{0, 0, 0}, da{0, 0, 0}, dq{0, 0, 0};
-    double mind, maxi;
-    __device__ __forceinline__ void add(const PairTerms &t) {
-        mind = checked_max_into(mind, t.rr);
-        maxi = checked_max_into(maxi, t.tp_dg);
-        ga += t.G;
-        da += t.dg;
-        dq += t.rxdg;
-        da += t.gj;
-        da -= t.lt1;
-        dq -= t.lt2;
-        da += t.lt3;
-        da += t.rad;
-    }
-};
 
 template <int BS>
 __global__ __launch_bounds__(BS) void k_main_pool(View d, const int2 *__restrict__ blk) {
@@ -1498,7 +1213,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 1 ?
             // wave 0 adds the first chains
             if (W > 1 && first_pass && t0 == 0 && wv == 1 && lane == 0) {
                 Harmonics h;
-                h.load(A, gi);
+                h = harmonics_load(A, gi);
                 const ddv3 gl = A.lddv(F_GL, gi);
                 hsh[0] = h.k2r; hsh[1] = h.scale; hsh[2] = h.R2; hsh[3] = h.A;
                 const double *c = &h.Cw.x.x;
@@ -1525,7 +1240,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 1 ?
                 }
                 GL = ddv3{dd{hsh[13], hsh[14]}, dd{hsh[15], hsh[16]}, dd{hsh[17], hsh[18]}};
             } else {
-                if (iters_left == 4) H.load(A, gi);
+                if (iters_left == 4) H = harmonics_load(A, gi);
                 GL = A.lddv(F_GL, gi);
             }
             bool brk;
@@ -1821,18 +1536,7 @@ __device__ __forceinline__ void stage_rotational(const Acc &A, int g, int k, dou
     }
     v3 egrad = A.ldv(F_EGRAD, g);
     if (k == 25) {
-        const m3 GI = A.ldm(F_GI, g);
-        const m3 ir = inverse(GI);
-        const double irdet = det(ir);
-        const double dt2 = dt * dt;
-        const v3 tq = tq_now * dt2;
-        const v3 tqdot = dt2 * (tq_now - egrad);
-        v3 j = todbl(A.lddv(F_GL, g));
-        const v3 w = w_now * dt;
-        const v3 irtq = ir * tq;
-        j = j * dt;
-        const v3 t1 = dot(w, j) * (ir * w + (GI * j) * irdet);
-        egrad = cross((t1 + cross(w, irtq)) + ir * ((cross(j, irtq) - 2 * cross(w, tq)) + tqdot), w);
+        egrad = egrad_k25(A.ldm(F_GI, g), todbl(A.lddv(F_GL, g)), w_now, tq_now, egrad, dt);
         A.stv(F_EGRAD, g, egrad);
     }
     const ddv3 gl0 = A.lddv(F_X0GL, g);
@@ -1904,7 +1608,7 @@ __device__ void deform_body_tile(const Acc &A, int gi, int il, int n, const JDef
     const double sR2 = A.ld(F_SR2, gi), k2 = A.ld(F_K2, gi), td = A.ld(F_TD, gi);
     const ddv3 GL = A.lddv(F_GL, gi);
     Harmonics H;
-    H.load(A, gi);
+    H = harmonics_load(A, gi);
     v3 w = A.ldv(F_W, gi);
     double phi = 0;
     v3 nacc{0, 0, 0};

@@ -5,19 +5,12 @@
 #include "../../include/ephemeris_hip.h"
 #include "eph_layout.hpp"
 #include "eph_math.hpp"
+#include "eph_terms.hpp"
 
 namespace eph {
 
 constexpr int kBS = 64;          // bodies per workgroup in the j-loop kernels (one wave64)
 constexpr int kSmallMax = 64;    // largest system integrated by one workgroup
-
-// partner data of one body as the main force reads it (also the record
-// format of a field system for test particles); 32 doubles
-struct JMain {
-    double rxh, rxl, ryh, ryl, rzh, rzl, GM, sR2;
-    double phi, beta2, bx, by, bz, nx, ny, nz;
-    double C[9], R2, GLx, GLy, GLz, lum, pad0, pad1;
-};
 
 // sum of the non-zero tableau weights of each stage (dt_k), index 1..25
 struct DtK { double v[26]; };

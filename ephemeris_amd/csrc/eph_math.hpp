@@ -116,6 +116,22 @@ EPH_HD double normalize2(double &pa, double &pb) {
     return f == 1 ? s : f * s;
 }
 EPH_HD v3 unit(v3 a) { normalize(a); return a; }
+// hypot_t<double>::length (vec_t.h:38-50): the norm, scaled for tiny/huge
+EPH_HD double length(v3 a) {
+    double x = a.x, y = a.y, z = a.z;
+    double r = x * x + y * y + z * z;
+    if (r < kHypLow) {
+        x *= kHypMul; y *= kHypMul; z *= kHypMul;
+        r = x * x + y * y + z * z;
+        return kHypDiv * sqrt(r);
+    }
+    if (!(r < kHypHigh)) {
+        x *= kHypDiv; y *= kHypDiv; z *= kHypDiv;
+        r = x * x + y * y + z * z;
+        return kHypMul * sqrt(r);
+    }
+    return sqrt(r);
+}
 EPH_HD v3 asc_node(v3 a) {                         // vec_t.h:226-230
     v3 p{-a.y, a.x, 0};
     normalize2(p.y, p.x);
@@ -195,6 +211,13 @@ EPH_HD v3 diff(ddv3 a, ddv3 b) {                   // fast_mpvec(a - b)
     return v3{todbl(sub(a.x, b.x)), todbl(sub(a.y, b.y)), todbl(sub(a.z, b.z))};
 }
 EPH_HD ddv3 scale(dd s, ddv3 a) { return ddv3{mul(s, a.x), mul(s, a.y), mul(s, a.z)}; }
+EPH_HD ddv3 sub(ddv3 a, ddv3 b) { return ddv3{sub(a.x, b.x), sub(a.y, b.y), sub(a.z, b.z)}; }
+// vec_t<dfloat_t> cross product (vec_t.h), every operation double-double
+EPH_HD ddv3 cross(ddv3 a, ddv3 b) {
+    return ddv3{sub(mul(a.y, b.z), mul(a.z, b.y)), sub(mul(a.z, b.x), mul(a.x, b.z)), sub(mul(a.x, b.y), mul(a.y, b.x))};
+}
+// vec_t<dfloat_t> / dfloat_t (vec_t.h:262-265): multiply by the reciprocal
+EPH_HD ddv3 divide(ddv3 a, dd s) { const dd rs = div(mkdd(1), s); return scale(rs, a); }
 EPH_HD bool finite(ddv3 a) { return finite(a.z) && finite(a.y) && finite(a.x); }
 EPH_HD m3 todbl(const ddm3 &a) { return m3{todbl(a.x), todbl(a.y), todbl(a.z)}; }
 EPH_HD ddm3 mkddm(const m3 &a) { return ddm3{mkddv(a.x), mkddv(a.y), mkddv(a.z)}; }

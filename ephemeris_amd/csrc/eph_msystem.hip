@@ -7,6 +7,10 @@
 #include <fstream>
 #include <sstream>
 
+#include <algorithm>
+#include <thread>
+
+#include "eph_cpu.hpp"
 #include "eph_math.hpp"
 #include "eph_runtime.hpp"
 
@@ -182,9 +186,66 @@ bool mass_sanity(const eph_mass &m, std::string *why) {   // mass_loader.cpp:18-
     return errs.empty();
 }
 
+MSystem::MSystem() = default;
+
 MSystem::~MSystem() {
-    if (sys) eph_system_destroy(sys);
+    drop_device();
 }
+
+void MSystem::drop_device() {
+    if (sys) eph_system_destroy(sys);
+    sys = nullptr;
+    gpu_stale = true;
+}
+
+void MSystem::reset_params() {                            // msystem::reset_params, mass_loader.cpp:296-308
+    t_eph_ = dd{0, 0};
+    delta_t_ = 300;
+    integrator_ = COMBINED_RK12;
+    data_cadence = 7200;
+    max_ephm_length = 631152000;
+    combined_delta_t = 28800;
+    GM_max_parent = 2E17;
+    GM_max_child = 1E13;
+    GM_max_tiny = 14E11;
+    Period_max_child = 0;
+}
+
+void MSystem::copy_params(const MSystem &o) {            // msystem::copy_params, mass_loader.cpp:924-937
+    t_eph_ = o.t_eph_;
+    delta_t_ = o.delta_t_;
+    integrator_ = o.integrator_;
+    data_cadence = o.data_cadence;
+    max_ephm_length = o.max_ephm_length;
+    combined_delta_t = o.combined_delta_t;
+    GM_max_child = o.GM_max_child;
+    GM_max_parent = o.GM_max_parent;
+    GM_max_tiny = o.GM_max_tiny;
+    Period_max_child = o.Period_max_child;
+}
+
+void MSystem::params(double *o) const {
+    o[0] = delta_t_; o[1] = data_cadence; o[2] = max_ephm_length; o[3] = combined_delta_t;
+    o[4] = GM_max_child; o[5] = GM_max_parent; o[6] = GM_max_tiny; o[7] = Period_max_child;
+}
+
+WorkerPool *MSystem::pool() {
+    if (!pool_) {
+        int n = n_threads;
+        if (n <= 0) n = (int)std::max(1u, std::thread::hardware_concurrency());
+        pool_.reset(new WorkerPool(n));
+    }
+    return pool_.get();
+}
+
+int MSystem::set_threads(int n) {
+    if (n < 0) { err = "thread count must be >= 0"; return fail(EPH_EINVAL, err); }
+    n_threads = n;
+    pool_.reset();
+    return EPH_OK;
+}
+
+int MSystem::threads() const { return pool_ ? pool_->threads() : n_threads; }
 
 int MSystem::load(const std::string &config_path, int summation_at_load) {
     LineReader rd(config_path);
@@ -192,6 +253,8 @@ int MSystem::load(const std::string &config_path, int summation_at_load) {
     static const char *keys[] = {"Initial", "Extra", "Geopotentials", "Rings", "TDB", "Integrator", "Delta_t",
                                  "Cadence", "Max_Ephemeris_Length", "Combined_Delta_t_Max", "Combined_GM_Max_Parent",
                                  "Combined_GM_Max_Child", "Combined_GM_Max_Tiny", "Combined_Period_Max_Child"};
+    enum { kInitial, kExtra, kGp, kRings, kTDB, kIntegrator, kDelta, kCadence, kMaxLen, kCDelta, kCParent, kCChild,
+           kCTiny, kCPeriod };
     std::vector<std::string> vals(sizeof keys / sizeof keys[0]);
     std::string line;
     std::vector<char> k(1100), v(1100);
@@ -204,14 +267,52 @@ int MSystem::load(const std::string &config_path, int summation_at_load) {
         if (!vals[q].empty()) { err = std::string("duplicate config parameter: ") + k.data(); return fail(EPH_EIO, err); }
         vals[q] = v.data();
     }
-    if (vals[0].empty()) { err = "config has no Initial"; return fail(EPH_EIO, err); }
+    if (vals[kInitial].empty()) { err = "config has no Initial"; return fail(EPH_EIO, err); }
     const std::string dir = dir_of(config_path) + "/";
-    int rc = load_files(dir + vals[0], vals[1].empty() ? "" : dir + vals[1], vals[2].empty() ? "" : dir + vals[2],
-                        vals[3].empty() ? "" : dir + vals[3], summation_at_load);
+    int rc = load_files(dir + vals[kInitial], vals[kExtra].empty() ? "" : dir + vals[kExtra],
+                        vals[kGp].empty() ? "" : dir + vals[kGp], vals[kRings].empty() ? "" : dir + vals[kRings],
+                        summation_at_load);
     if (rc) return rc;
-    if (!vals[4].empty()) { t_eph[0] = (double)atoll(vals[4].c_str()); t_eph[1] = 0; }
-    if (!vals[6].empty()) delta_t_ = atof(vals[6].c_str());
-    if (!vals[4].empty()) return eph_system_upload(sys, mlist.data(), t_eph);
+    if (!vals[kIntegrator].empty()) {                     // mass_loader.cpp:188-204
+        static const char *names[] = {"CPU_RK12", "GPU_RK12", "COMBINED_RK12"};
+        integrator_ = -1;
+        for (int q = 0; q < 3; ++q)
+            if (vals[kIntegrator] == names[q]) integrator_ = q;
+        if (integrator_ < 0) { err = "invalid integrator: " + vals[kIntegrator]; return fail(EPH_EIO, err); }
+    }
+    auto load_param = [&](int key, double &param) {       // LOAD_CONFIG, mass_loader.cpp:206-211
+        if (!vals[key].empty()) param = atof(vals[key].c_str());
+    };
+    double tdb = 0;
+    load_param(kTDB, tdb);
+    load_param(kDelta, delta_t_);
+    load_param(kCadence, data_cadence);
+    load_param(kMaxLen, max_ephm_length);
+    if (integrator_ == COMBINED_RK12) {
+        load_param(kCDelta, combined_delta_t);
+        load_param(kCParent, GM_max_parent);
+        load_param(kCChild, GM_max_child);
+        load_param(kCTiny, GM_max_tiny);
+        load_param(kCPeriod, Period_max_child);
+    }
+    t_eph_ = dd{tdb, 0};
+    if (!(std::fabs(tdb) < 0x1p63) || (double)(int64_t)tdb != tdb) {   // mass_loader.cpp:226-229
+        err = "TDB (initial ephemeris time, seconds) should be an integer";
+        return fail(EPH_EIO, err);
+    }
+    for (auto &m : mlist) {                               // correct time variables to t = 0, mass_loader.cpp:231-237
+        m.GM0 = m.GM - m.dGM * tdb;
+        m.exJ2 = m.dJ2 * tdb;
+        m.C_static.x.x -= m.exJ2 / 2;
+        m.C_static.y.y -= m.exJ2 / 2;
+        m.C_static.z.z += m.exJ2;
+    }
+    auto positive = [](double x) { return x > 0 && x != INFINITY; };   // mass_loader.cpp:14-16, 239-243
+    if (!positive(delta_t_) || !positive(data_cadence) || !positive(max_ephm_length) || !positive(combined_delta_t)) {
+        err = "Delta_t/Cadence/Max_Ephemeris_Length/Combined_Delta_t_Max should be finite positive real numbers";
+        return fail(EPH_EIO, err);
+    }
+    host_changed();
     return EPH_OK;
 }
 
@@ -221,7 +322,11 @@ int MSystem::load_files(const std::string &initial, const std::string &extra, co
     gps.clear();
     rgs.clear();
     diags.clear();
-    if (sys) { eph_system_destroy(sys); sys = nullptr; }
+    blist.clear();
+    tidal_children.clear();
+    tidal_parent = -1;
+    drop_device();
+    reset_params();                                       // mass_loader.cpp:329
     LineReader rd(initial);
     if (!rd.f) { err = initial + ": file not found"; return fail(EPH_EIO, err); }
     std::string line;
@@ -314,19 +419,14 @@ int MSystem::load_files(const std::string &initial, const std::string &extra, co
 }
 
 int MSystem::finish_load(int summation_at_load) {
-    const int64_t n = size();
-    const double t0[2] = {t_eph[0], t_eph[1]};
-    int rc = eph_system_create(1, &n, mlist.data(), gps.data(), (int64_t)gps.size(), rgs.data(), (int64_t)rgs.size(),
-                               t0, &sys);
-    if (rc) return rc;
-    if ((rc = eph_deform(sys))) return rc;                // msystem::deform, mass_loader.cpp:541
-    if ((rc = eph_system_download(sys, mlist.data(), nullptr))) return rc;
+    const CpuSystem cs = cpu_view();
+    cpu_deform(cs, pool());                               // msystem::deform, mass_loader.cpp:541
     std::string why;
     for (auto &m : mlist) {                               // mass_loader.cpp:544-551
         const m3 fmis = todbl(DM(m.s));
-        const m3 cs = M(m.C_static);
-        const m3 ncp = toworld(fmis, cs);
-        m.C_static = E(cs - (tolocal(fmis, M(m.C_potential)) - cs));
+        const m3 cs_ = M(m.C_static);
+        const m3 ncp = toworld(fmis, cs_);
+        m.C_static = E(cs_ - (tolocal(fmis, M(m.C_potential)) - cs_));
         m.C_potential = E(ncp);
         std::string w;
         if (!mass_sanity(m, &w)) {
@@ -336,32 +436,102 @@ int MSystem::finish_load(int summation_at_load) {
         }
     }
     if (!why.empty()) { err = "sanity check failed: " + why; return fail(EPH_ESANITY, err); }
-    if ((rc = eph_system_upload(sys, mlist.data(), t0))) return rc;
-    return accel(summation_at_load == EPH_SUMMATION_SERIAL ? -1 : 0);
+    int rc = cpu_accel_now(summation_at_load == EPH_SUMMATION_SERIAL ? -1 : 0);   // mass_loader.cpp:555
+    if (rc) return rc;
+    analyse(true);                                        // mass_loader.cpp:558
+    return EPH_OK;
+}
+
+int MSystem::assign(int64_t n, const eph_mass *masses, const eph_geopotential *g, int64_t n_gp, const eph_ring *r,
+                    int64_t n_ring, dd t) {
+    if (n <= 0 || !masses) { err = "no bodies"; return fail(EPH_EINVAL, err); }
+    for (int64_t i = 0; i < n; ++i)
+        if (masses[i].gp >= n_gp || masses[i].ring >= n_ring) { err = "model index out of range"; return fail(EPH_EINVAL, err); }
+    drop_device();
+    mlist.assign(masses, masses + n);
+    gps.assign(g, g + (g ? n_gp : 0));
+    rgs.assign(r, r + (r ? n_ring : 0));
+    for (auto &m : mlist) {
+        if (m.gp < 0) m.gp = -1;
+        if (m.ring < 0) m.ring = -1;
+    }
+    t_eph_ = t;
+    diags.clear();
+    blist.clear();
+    t_barycen = t_update = dd{NAN, NAN};
+    tidal_children.clear();
+    tidal_parent = -1;
+    return EPH_OK;
+}
+
+int MSystem::set_masses(const eph_mass *m, int64_t n) {
+    if (n != size() || !m) { err = "body count differs from the system's"; return fail(EPH_EINVAL, err); }
+    for (int64_t i = 0; i < n; ++i)
+        if (m[i].gp != mlist[(size_t)i].gp || m[i].ring != mlist[(size_t)i].ring) {
+            err = "model indices differ from the system's";
+            return fail(EPH_EINVAL, err);
+        }
+    std::copy(m, m + n, mlist.begin());
+    host_changed();
+    return EPH_OK;
+}
+
+CpuSystem MSystem::cpu_view() {
+    CpuSystem cs;
+    cs.m = mlist.data();
+    cs.n = size();
+    cs.gps = gps.data();
+    cs.rings = rgs.data();
+    cs.tidal_parent = tidal_parent;
+    cs.tidal_matrix = tidal_matrix;
+    cs.tidal_children = tidal_children.data();
+    cs.n_tidal_children = (int64_t)tidal_children.size();
+    return cs;
+}
+
+int MSystem::ensure_device() {
+    if (!sys) {
+        const int64_t n = size();
+        const double t0[2] = {t_eph_.hi, t_eph_.lo};
+        int rc = eph_system_create(1, &n, mlist.data(), gps.data(), (int64_t)gps.size(), rgs.data(),
+                                   (int64_t)rgs.size(), t0, &sys);
+        if (rc) { sys = nullptr; return rc; }
+        gpu_stale = false;
+    } else if (gpu_stale) {
+        const double t0[2] = {t_eph_.hi, t_eph_.lo};
+        int rc = eph_system_upload(sys, mlist.data(), t0);
+        if (rc) return rc;
+        gpu_stale = false;
+    }
+    return EPH_OK;
 }
 
 int MSystem::sync_down() {
-    return eph_system_download(sys, mlist.data(), t_eph);
+    double t[2];
+    int rc = eph_system_download(sys, mlist.data(), t);
+    if (rc) return rc;
+    t_eph_ = dd{t[0], t[1]};
+    return EPH_OK;
+}
+
+int MSystem::cpu_accel_now(int parallel_option) {
+    const CpuSystem cs = cpu_view();
+    cpu_accel(cs, parallel_option, pool());
+    host_changed();
+    return EPH_OK;
 }
 
 int MSystem::accel(int parallel_option) {
-    if (!sys) { err = "no system loaded"; return fail(EPH_EINVAL, err); }
-    int rc = eph_accel(sys, parallel_option < 0 ? EPH_SUMMATION_SERIAL : EPH_SUMMATION_POOL);
+    if (mlist.empty()) { err = "no system loaded"; return fail(EPH_EINVAL, err); }
+    if (parallel_option <= 0) return cpu_accel_now(parallel_option);
+    int rc = ensure_device();                             // msystem::Cuda_accel, mass.cpp:218-221
     if (rc) return rc;
+    if ((rc = eph_accel(sys, EPH_SUMMATION_POOL))) return rc;
     return sync_down();
 }
 
-int MSystem::integrate(double dt, int64_t n_step, int use_gpu) {
-    if (!sys) { err = "no system loaded"; return fail(EPH_EINVAL, err); }
-    if (use_gpu != GPU_RK12) {
-        err = "only the GPU back end (USE_GPU=1) is provided by this library";
-        return fail(EPH_ENOTSUP, err);
-    }
-    if (n_step == 0) return EPH_OK;
-    int rc = eph_integrate(sys, dt, n_step);
-    if (rc) return rc;
-    if ((rc = sync_down())) return rc;
-    // collision and step-size diagnostics, mass.cpp:411-439
+// collision and step-size diagnostics, mass.cpp:411-439
+void MSystem::post_integrate(double dt) {
     diags.clear();
     const double TIMESTEP_THRESHOLD = 0.01;
     const double dt2 = dt * dt;
@@ -371,21 +541,127 @@ int MSystem::integrate(double dt, int64_t n_step, int use_gpu) {
         if (m.max_influence * dt2 > TIMESTEP_THRESHOLD)
             diags.push_back(Diagnostic{i, 1, std::abs(dt), std::sqrt(TIMESTEP_THRESHOLD / m.max_influence)});
     }
+}
+
+int MSystem::integrate(double dt, int64_t n_step, int use_gpu) {
+    if (mlist.empty()) { err = "no system loaded"; return fail(EPH_EINVAL, err); }
+    if (!(dt == dt) || dt == 0) { err = "dt must be finite and non-zero"; return fail(EPH_EINVAL, err); }
+    if (n_step == 0) return EPH_OK;
+    if (use_gpu != CPU_RK12 && use_gpu != GPU_RK12) { err = "USE_GPU must be 0 or 1"; return fail(EPH_EINVAL, err); }
+    const double dts = n_step < 0 ? -dt : dt;
+    if (use_gpu == CPU_RK12) {
+        const CpuSystem cs = cpu_view();
+        cpu_integrate(cs, t_eph_, dt, n_step, pool());
+        host_changed();
+    } else {
+        int rc = ensure_device();
+        if (rc) return rc;
+        if ((rc = eph_integrate(sys, dt, n_step))) return rc;
+        if ((rc = sync_down())) return rc;
+    }
+    post_integrate(dts);
     return EPH_OK;
 }
 
-int64_t MSystem::get_mid(const char *ssid) const {
+void MSystem::update(double t) {                          // msystem::update(t), mass.cpp:28-48
+    const CpuSystem cs = cpu_view();
+    cpu_update(cs, t);
+    host_changed();
+}
+
+int MSystem::scale(int64_t mid, double factor) {         // mass::scale, mass.cpp:8-26
+    if (mid < 0 || mid >= size()) { err = "body index out of range"; return fail(EPH_EINVAL, err); }
+    eph_mass &m = mlist[(size_t)mid];
+    const double f2 = factor * factor, f3 = factor * f2;
+    m.R *= factor;
+    m.R2 *= f2;
+    m.sR2 *= f2;
+    m.GL = E(eph::scale(mkdd(f2), DV(m.GL)));
+    m.GI = E(f2 * M(m.GI));
+    m.GM *= f3;
+    m.GM0 *= f3;
+    m.dGM *= f3;
+    m.lum *= f2;
+    m.recpt *= factor;
+    m.rR2G_4c *= f3;
+    host_changed();
+    return EPH_OK;
+}
+
+int MSystem::scale_geopotential(int64_t mid, double factor) {   // mass_loader.cpp:1008-1021
+    if (mid < 0 || mid >= size()) { err = "body index out of range"; return fail(EPH_EINVAL, err); }
+    eph_mass &mi = mlist[(size_t)mid];
+    if (mi.gp >= 0 && factor != 0) {                      // geopotential::copy(gp, factor), geopotential.cpp:154-166
+        eph_geopotential g = gps[(size_t)mi.gp];
+        if (factor != 1) {
+            const int64_t nt = g.Nz * (g.Nz + 2) - 3;
+            for (int64_t q = 0; q < nt; ++q) g.c[q] *= factor;
+        }
+        mi.gp = (int64_t)gps.size();
+        gps.push_back(g);
+    } else
+        mi.gp = -1;
+    mi.C_potential = E(factor * M(mi.C_potential));
+    mi.C_static = E(factor * M(mi.C_static));
+    mi.k2 *= factor;
+    mi.k2r *= factor;
+    mi.exJ2 *= factor;
+    mi.dJ2 *= factor;
+    const m3 GI = (-2.0 / 3 * mi.R2) * subdiag(M(mi.C_potential), mi.A);
+    mi.GI = E(GI);
+    mi.GL = E(mkddv(GI * V(mi.w)));
+    drop_device();
+    return EPH_OK;
+}
+
+int MSystem::scale_ring(int64_t mid, double factor) {    // mass_loader.cpp:1022-1026
+    if (mid < 0 || mid >= size()) { err = "body index out of range"; return fail(EPH_EINVAL, err); }
+    eph_mass &mi = mlist[(size_t)mid];
+    if (mi.ring >= 0 && factor != 0) {                    // ring::copy(rp, factor), ring.cpp:118-134
+        eph_ring r = rgs[(size_t)mi.ring];
+        if (factor != 1) {
+            const double host_ratio = 1 - r.GM_ratio;
+            const double new_ratio = r.GM_ratio * factor;
+            r.GM_ratio = new_ratio / (host_ratio + new_ratio);
+            r.A *= factor;
+            r.J2 *= factor;
+            r.GL_R2 *= factor;
+            for (int64_t q = 0; q < r.N; ++q) r.GsR2[q] *= factor;
+        }
+        mi.ring = (int64_t)rgs.size();
+        rgs.push_back(r);
+    } else
+        mi.ring = -1;
+    drop_device();
+    return EPH_OK;
+}
+
+int MSystem::set_tidal(int64_t parent, const m3 &matrix, const std::vector<int64_t> &children) {
+    for (int64_t c : children)
+        if (c < 0 || c >= size() || c == parent) { err = "tidal child index out of range"; return fail(EPH_EINVAL, err); }
+    if (!children.empty() && (parent < 0 || parent >= size())) { err = "tidal parent index out of range"; return fail(EPH_EINVAL, err); }
+    tidal_parent = parent;
+    tidal_matrix = matrix;
+    tidal_children = children;
+    return EPH_OK;
+}
+
+int64_t MSystem::get_mid(const char *ssid) const {      // mass_loader.cpp:901-908
     uint64_t key = 0;
     const size_t sl = strlen(ssid);
     if (sl >= 8) return -1;
     memcpy(&key, ssid, sl);
+    return get_mid(key);
+}
+
+int64_t MSystem::get_mid(uint64_t sid) const {
     for (size_t i = 0; i < mlist.size(); ++i)
-        if (mlist[i].sid == key) return (int64_t)i;
+        if (mlist[i].sid == sid) return (int64_t)i;
     return -1;
 }
 
 int64_t MSystem::ephemeris_time() const {                // dfloat_t operator I, dfloat_t.h:62-67
-    const dd t{t_eph[0], t_eph[1]};
+    const dd t = t_eph_;
     int64_t ia = (int64_t)std::llround(t.hi + t.lo);
     const dd di{(double)ia, (double)(ia - (int64_t)(double)ia)};
     return ia + (int64_t)std::floor(todbl(sub(t, di)));
@@ -411,6 +687,30 @@ int eph_msystem_load(const char *config_path, int summation_at_load, eph_msystem
     return EPH_OK;
 }
 
+int eph_msystem_create(int64_t n, const eph_mass *masses, const eph_geopotential *gps, int64_t n_gp,
+                       const eph_ring *rings, int64_t n_ring, const double *t_eph, eph_msystem **out) {
+    if (!masses || !out || n <= 0) return eph::fail(EPH_EINVAL, "null or empty argument");
+    auto *m = new MSystem();
+    int rc = m->assign(n, masses, gps, n_gp, rings, n_ring, t_eph ? eph::dd{t_eph[0], t_eph[1]} : eph::dd{0, 0});
+    if (rc) { delete m; return rc; }
+    *out = reinterpret_cast<eph_msystem *>(m);
+    return EPH_OK;
+}
+
+int eph_msystem_load_checkpoint(const char *path, eph_msystem **out) {
+    if (!path || !out) return eph::fail(EPH_EINVAL, "null argument");
+    auto *m = new MSystem();
+    int rc = m->load_checkpoint(std::string(path));
+    if (rc) { delete m; return rc; }
+    *out = reinterpret_cast<eph_msystem *>(m);
+    return EPH_OK;
+}
+
+int eph_msystem_save_checkpoint(const eph_msystem *h, const char *path) {
+    if (!h || !path) return eph::fail(EPH_EINVAL, "null argument");
+    return H(h)->save_checkpoint(std::string(path));
+}
+
 int eph_msystem_destroy(eph_msystem *h) {
     if (!h) return eph::fail(EPH_EINVAL, "null handle");
     delete H(h);
@@ -424,6 +724,11 @@ int eph_msystem_masses(const eph_msystem *h, eph_mass *out, int64_t cap) {
     if (cap < H(h)->size()) return eph::fail(EPH_EINVAL, "output capacity too small");
     for (int64_t i = 0; i < H(h)->size(); ++i) out[i] = (*H(h))[i];
     return EPH_OK;
+}
+
+int eph_msystem_set_masses(eph_msystem *h, const eph_mass *masses, int64_t n) {
+    if (!h || !masses) return eph::fail(EPH_EINVAL, "null argument");
+    return H(h)->set_masses(masses, n);
 }
 
 int eph_msystem_models(const eph_msystem *h, eph_geopotential *gps, int64_t gcap, eph_ring *rings, int64_t rcap,
@@ -440,8 +745,21 @@ int eph_msystem_models(const eph_msystem *h, eph_geopotential *gps, int64_t gcap
 
 int eph_msystem_time(const eph_msystem *h, double *t_eph) {
     if (!h || !t_eph) return eph::fail(EPH_EINVAL, "null argument");
-    t_eph[0] = H(h)->t_eph_hi();
-    t_eph[1] = H(h)->t_eph_lo();
+    t_eph[0] = H(h)->t_eph().hi;
+    t_eph[1] = H(h)->t_eph().lo;
+    return EPH_OK;
+}
+
+int eph_msystem_set_time(eph_msystem *h, const double *t_eph) {
+    if (!h || !t_eph) return eph::fail(EPH_EINVAL, "null argument");
+    H(h)->set_t_eph(eph::dd{t_eph[0], t_eph[1]});
+    return EPH_OK;
+}
+
+int eph_msystem_params(const eph_msystem *h, double *params8, int64_t *integrator) {
+    if (!h) return eph::fail(EPH_EINVAL, "null handle");
+    if (params8) H(h)->params(params8);
+    if (integrator) *integrator = H(h)->integrator();
     return EPH_OK;
 }
 
@@ -453,6 +771,83 @@ int eph_msystem_accel(eph_msystem *h, int parallel_option) {
 int eph_msystem_integrate(eph_msystem *h, double dt, int64_t n_step, int use_gpu) {
     if (!h) return eph::fail(EPH_EINVAL, "null handle");
     return H(h)->integrate(dt, n_step, use_gpu);
+}
+
+int eph_msystem_combined_integrate(eph_msystem *h, double dt, int64_t n_combine, int64_t n_step, int use_gpu) {
+    if (!h) return eph::fail(EPH_EINVAL, "null handle");
+    return H(h)->combined_integrate(dt, n_combine, n_step, use_gpu);
+}
+
+int eph_msystem_update(eph_msystem *h, double t) {
+    if (!h) return eph::fail(EPH_EINVAL, "null handle");
+    H(h)->update(t);
+    return EPH_OK;
+}
+
+int eph_msystem_analyse(eph_msystem *h, int reconstruct, double *t_update) {
+    if (!h) return eph::fail(EPH_EINVAL, "null handle");
+    const eph::dd t = H(h)->analyse(reconstruct != 0);
+    if (t_update) { t_update[0] = t.hi; t_update[1] = t.lo; }
+    if (!H(h)->analysis_ok())
+        return eph::fail(EPH_EINVAL, "orbit-structure analysis did not settle; the previous structure is kept");
+    return EPH_OK;
+}
+
+int64_t eph_msystem_barycens(const eph_msystem *h, int64_t *ids, double *gm_sys, int64_t *children, int64_t cap,
+                             int64_t child_cap) {
+    if (!h) return eph::fail(EPH_EINVAL, "null handle");
+    const auto &bl = H(h)->barycens();
+    int64_t nc = 0;
+    for (int64_t i = 0; i < (int64_t)bl.size() && i < cap; ++i) {
+        const eph::Barycen &b = bl[(size_t)i];
+        if (ids) {
+            int64_t *o = ids + 7 * i;
+            o[0] = b.pid; o[1] = b.hid; o[2] = b.gid; o[3] = b.tid; o[4] = b.mid;
+            o[5] = (int64_t)b.children.size();
+            o[6] = nc;
+        }
+        if (gm_sys) { gm_sys[2 * i] = b.GM_sys.hi; gm_sys[2 * i + 1] = b.GM_sys.lo; }
+        for (int64_t c : b.children) {
+            if (children && nc < child_cap) children[nc] = c;
+            ++nc;
+        }
+    }
+    return (int64_t)bl.size();
+}
+
+int eph_msystem_scale(eph_msystem *h, int64_t mid, double factor) {
+    if (!h) return eph::fail(EPH_EINVAL, "null handle");
+    return H(h)->scale(mid, factor);
+}
+
+int eph_msystem_scale_geopotential(eph_msystem *h, int64_t mid, double factor) {
+    if (!h) return eph::fail(EPH_EINVAL, "null handle");
+    return H(h)->scale_geopotential(mid, factor);
+}
+
+int eph_msystem_scale_ring(eph_msystem *h, int64_t mid, double factor) {
+    if (!h) return eph::fail(EPH_EINVAL, "null handle");
+    return H(h)->scale_ring(mid, factor);
+}
+
+int eph_msystem_set_tidal(eph_msystem *h, int64_t parent, const double *matrix9, const int64_t *children,
+                          int64_t n_children) {
+    if (!h || (n_children > 0 && (!children || !matrix9))) return eph::fail(EPH_EINVAL, "null argument");
+    eph::m3 m{};
+    if (matrix9) m = eph::m3{eph::v3{matrix9[0], matrix9[1], matrix9[2]}, eph::v3{matrix9[3], matrix9[4], matrix9[5]},
+                             eph::v3{matrix9[6], matrix9[7], matrix9[8]}};
+    std::vector<int64_t> ch(children, children + (n_children > 0 ? n_children : 0));
+    return H(h)->set_tidal(parent, m, ch);
+}
+
+int eph_msystem_set_threads(eph_msystem *h, int n_threads) {
+    if (!h) return eph::fail(EPH_EINVAL, "null handle");
+    return H(h)->set_threads(n_threads);
+}
+
+int64_t eph_msystem_get_mid(const eph_msystem *h, const char *ssid) {
+    if (!h || !ssid) return eph::fail(EPH_EINVAL, "null argument");
+    return H(h)->get_mid(ssid);
 }
 
 int eph_msystem_diagnostics(const eph_msystem *h, int64_t *body, int *kind, double *value, double *limit, int64_t cap) {

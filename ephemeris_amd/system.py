@@ -113,16 +113,40 @@ class GpuSystems:
 
 
 class MSystem:
-    """A system loaded from the reference's text inputs (msystem of the
-    reference: load / accel / integrate), resident on the GPU."""
+    """A system in the msystem interface of the reference: loaded from the
+    reference's text inputs or a checkpoint, or built from bodies in memory;
+    accel / integrate on the CPU back end or on the GPU (parallel_option 1 /
+    USE_GPU 1), combined integration, orbit-structure analysis."""
 
-    CPU_RK12, GPU_RK12 = 0, 1
+    CPU_RK12, GPU_RK12, COMBINED_RK12 = 0, 1, 2
 
-    def __init__(self, config_path, summation_at_load=_lib.SUMMATION_SERIAL, device=0):
-        _lib.init(device)
-        h = C.c_void_p()
-        _lib.check(_lib.lib().eph_msystem_load(str(config_path).encode(), summation_at_load, C.byref(h)))
+    def __init__(self, config_path=None, summation_at_load=_lib.SUMMATION_SERIAL, *, handle=None, threads=0):
+        h = C.c_void_p() if handle is None else handle
+        if handle is None:
+            _lib.check(_lib.lib().eph_msystem_load(str(config_path).encode(), summation_at_load, C.byref(h)))
         self.h = h
+        if threads:
+            self.set_threads(threads)
+
+    @classmethod
+    def from_masses(cls, masses, gps=None, rings=None, t_eph=(0.0, 0.0), threads=0):
+        masses = np.ascontiguousarray(masses, dtype=MASS_DTYPE)
+        gps = np.ascontiguousarray(np.zeros(0, GP_DTYPE) if gps is None else gps, dtype=GP_DTYPE)
+        rings = np.ascontiguousarray(np.zeros(0, RING_DTYPE) if rings is None else rings, dtype=RING_DTYPE)
+        t = (C.c_double * 2)(*t_eph)
+        h = C.c_void_p()
+        _lib.check(_lib.lib().eph_msystem_create(len(masses), _ptr(masses), _ptr(gps), len(gps), _ptr(rings),
+                                                 len(rings), t, C.byref(h)))
+        return cls(handle=h, threads=threads)
+
+    @classmethod
+    def from_checkpoint(cls, path, threads=0):
+        h = C.c_void_p()
+        _lib.check(_lib.lib().eph_msystem_load_checkpoint(str(path).encode(), C.byref(h)))
+        return cls(handle=h, threads=threads)
+
+    def save_checkpoint(self, path):
+        _lib.check(_lib.lib().eph_msystem_save_checkpoint(self.h, str(path).encode()))
 
     def close(self):
         if getattr(self, "h", None):
@@ -144,6 +168,10 @@ class MSystem:
         _lib.check(_lib.lib().eph_msystem_masses(self.h, _ptr(m), len(m)))
         return m
 
+    def set_masses(self, masses):
+        masses = np.ascontiguousarray(masses, dtype=MASS_DTYPE)
+        _lib.check(_lib.lib().eph_msystem_set_masses(self.h, _ptr(masses), len(masses)))
+
     def models(self):
         ng, nr = C.c_int64(), C.c_int64()
         _lib.check(_lib.lib().eph_msystem_models(self.h, None, 0, None, 0, C.byref(ng), C.byref(nr)))
@@ -156,11 +184,75 @@ class MSystem:
         _lib.check(_lib.lib().eph_msystem_time(self.h, t))
         return t[0], t[1]
 
+    def set_t_eph(self, t):
+        _lib.check(_lib.lib().eph_msystem_set_time(self.h, (C.c_double * 2)(*t)))
+
+    def params(self):
+        p = (C.c_double * 8)()
+        integ = C.c_int64()
+        _lib.check(_lib.lib().eph_msystem_params(self.h, p, C.byref(integ)))
+        names = ("delta_t", "data_cadence", "max_ephm_length", "combined_delta_t", "GM_max_child",
+                 "GM_max_parent", "GM_max_tiny", "Period_max_child")
+        d = dict(zip(names, p))
+        d["integrator"] = integ.value
+        return d
+
     def accel(self, parallel_option=0):
-        _lib.check(_lib.lib().eph_msystem_accel(self.h, parallel_option))
+        """1: GPU; 0: CPU threads (thread-pool order when N is large); -1: CPU single thread."""
+        if parallel_option > 0:
+            _lib.init()
+        _lib.check(_lib.lib().eph_msystem_accel(self.h, int(parallel_option)))
 
     def integrate(self, dt, n_step, use_gpu=1):
+        """USE_GPU 1: GPU RK12; 0: CPU RK12."""
+        if use_gpu:
+            _lib.init()
         _lib.check(_lib.lib().eph_msystem_integrate(self.h, float(dt), int(n_step), int(use_gpu)))
+
+    def combined_integrate(self, dt, n_combine, n_step, use_gpu=1):
+        if use_gpu:
+            _lib.init()
+        _lib.check(_lib.lib().eph_msystem_combined_integrate(self.h, float(dt), int(n_combine), int(n_step),
+                                                             int(use_gpu)))
+
+    def update(self, t):
+        _lib.check(_lib.lib().eph_msystem_update(self.h, float(t)))
+
+    def analyse(self, reconstruct=False):
+        t = (C.c_double * 2)()
+        _lib.check(_lib.lib().eph_msystem_analyse(self.h, int(bool(reconstruct)), t))
+        return t[0], t[1]
+
+    def barycens(self):
+        """[(pid, hid, gid, tid, mid, children, GM_sys)] of the orbit hierarchy."""
+        n = _lib.check(_lib.lib().eph_msystem_barycens(self.h, None, None, None, 0, 0))
+        ids = np.zeros((max(n, 1), 7), np.int64)
+        gm = np.zeros((max(n, 1), 2))
+        nch = n * n + 1
+        ch = np.zeros(nch, np.int64)
+        _lib.check(_lib.lib().eph_msystem_barycens(self.h, _ptr(ids), _ptr(gm), _ptr(ch), n, nch))
+        return [(int(a[0]), int(a[1]), int(a[2]), int(a[3]), int(a[4]),
+                 [int(c) for c in ch[a[6]:a[6] + a[5]]], gm[i, 0] + gm[i, 1]) for i, a in enumerate(ids[:n])]
+
+    def scale(self, mid, factor):
+        _lib.check(_lib.lib().eph_msystem_scale(self.h, int(mid), float(factor)))
+
+    def scale_geopotential(self, mid, factor):
+        _lib.check(_lib.lib().eph_msystem_scale_geopotential(self.h, int(mid), float(factor)))
+
+    def scale_ring(self, mid, factor):
+        _lib.check(_lib.lib().eph_msystem_scale_ring(self.h, int(mid), float(factor)))
+
+    def set_tidal(self, parent, matrix, children):
+        m = np.ascontiguousarray(matrix, np.float64).reshape(9)
+        ch = np.ascontiguousarray(children, np.int64)
+        _lib.check(_lib.lib().eph_msystem_set_tidal(self.h, int(parent), _ptr(m), _ptr(ch), len(ch)))
+
+    def set_threads(self, n):
+        _lib.check(_lib.lib().eph_msystem_set_threads(self.h, int(n)))
+
+    def get_mid(self, ssid):
+        return int(_lib.lib().eph_msystem_get_mid(self.h, ssid.encode()))
 
     def diagnostics(self):
         n = _lib.check(_lib.lib().eph_msystem_diagnostics(self.h, None, None, None, None, 0))

@@ -16,7 +16,7 @@
  *
  * Reference interfaces replaced (paths relative to the reference checkout):
  *   eph_accel       <- msystem::Cuda_accel()        src/integrators/RungeKutta.cu:418-438
- *                      msystem::accel(int)          src/physics/mass.cpp:216-397
+ *                      (GPU twin of msystem::accel(int), src/physics/mass.cpp:216-397)
  *   eph_integrate   <- msystem::Cuda_RungeKutta12() src/integrators/RungeKutta.cu:391-412
  *                      msystem::integrate(dt,n,1)   src/physics/mass.cpp:399-442
  *   eph_deform      <- msystem::deform()            src/physics/mass.cpp:104-110
@@ -249,27 +249,73 @@ int eph_profile_classes(eph_system *sys, unsigned mask);
 int eph_profile_read(eph_system *sys, const char **names, double *avg_ms, int64_t *launches, int cap);
 int eph_profile_reset(eph_system *sys);
 
-/* ---- host system object (text inputs -> resident system) ------------------
- * The msystem operator interface of the reference around the calls above:
- *   eph_msystem_load       <- msystem::load(fconfig)      src/modules/mass_loader.cpp:67-294, 324-561
- *   eph_msystem_accel      <- msystem::accel(int)         src/physics/mass.cpp:216-397
- *   eph_msystem_integrate  <- msystem::integrate(dt,n,USE_GPU) src/physics/mass.cpp:399-442
- * USE_GPU must be 1 (GPU_RK12): no CPU back end is provided (EPH_ENOTSUP).
- * parallel_option -1 selects the single-thread summation order, 0 and 1 the
- * thread-pool order.  Diagnostics mirror the collision (kind 0: value = the
- * closest approach, limit = R) and step-size (kind 1: value = |dt|, limit =
- * recommended dt) reports of mass.cpp:411-439; eph_msystem_diagnostics
- * returns their number and copies up to cap of them. */
+/* ---- host system object (the msystem operator interface) ------------------
+ * The msystem of the reference around the calls above; the host copy of the
+ * bodies is authoritative, the GPU keeps a resident copy refreshed only when
+ * the host copy changed.
+ *   eph_msystem_load            <- msystem::load(fconfig)   src/modules/mass_loader.cpp:67-294, 324-561
+ *                                  (TDB correction :225-237; initial deform, C_static adjustment,
+ *                                  accel and analyse(true) :540-558, on the CPU back end)
+ *   eph_msystem_create          <- msystem::push_back of each body + copy_params (mass_loader.cpp:924-1004)
+ *   eph_msystem_load_checkpoint <- msystem::load_checkpoint  mass_loader.cpp:614-698
+ *   eph_msystem_save_checkpoint <- msystem::save_checkpoint  mass_loader.cpp:715-759
+ *                                  (binary layout of the reference: header, barycen ids, per body the
+ *                                  824 bytes of class mass up to Egrad with model pointers replaced
+ *                                  by the models' byte counts, then the geopotential and ring objects)
+ *   eph_msystem_accel           <- msystem::accel(int)       src/physics/mass.cpp:216-397
+ *                                  parallel_option 1 = GPU (msystem::Cuda_accel, RungeKutta.cu:418-438),
+ *                                  0 = CPU threads (the pool order when min(N, ceil(N^2/2304)) > 1),
+ *                                  -1 = CPU single-thread order; the CPU accel applies the tidal-matrix
+ *                                  correction (mass.cpp:376-387), the GPU accel does not (as in the reference)
+ *   eph_msystem_integrate       <- msystem::integrate(dt,n,USE_GPU) mass.cpp:399-442
+ *                                  USE_GPU 0 = CPU RK12 (RungeKutta.impl:107-194), 1 = GPU RK12
+ *   eph_msystem_combined_integrate <- msystem::combined_integrate src/integrators/mass_combined.cpp:40-406
+ *   eph_msystem_update          <- msystem::update(t)        mass.cpp:28-48
+ *   eph_msystem_analyse         <- msystem::analyse(reconstruct) src/physics/barycen_analyzer.cpp:12-310
+ *   eph_msystem_scale / _scale_geopotential / _scale_ring
+ *                               <- msystem::scale / scale_geopotential / scale_ring, mass_loader.cpp:1005-1026
+ *   eph_msystem_set_tidal       <- the tidal_parent / tidal_matrix / tidal_childlist members (mass.h:236-240)
+ *                                  that combined_integrate sets on its subsystems
+ * Diagnostics mirror the collision (kind 0: value = the closest approach,
+ * limit = R) and step-size (kind 1: value = |dt|, limit = recommended dt)
+ * reports of mass.cpp:411-439; eph_msystem_diagnostics returns their number
+ * and copies up to cap of them.  eph_msystem_barycens writes per barycen
+ * {pid, hid, gid, tid, mid, number of children, offset of its children in
+ * `children`} (7 int64), GM_sys as {hi, lo}, and the children lists, and
+ * returns the number of barycens. */
 typedef struct eph_msystem eph_msystem;
 int eph_msystem_load(const char *config_path, int summation_at_load, eph_msystem **out);
+int eph_msystem_create(int64_t n, const eph_mass *masses, const eph_geopotential *gps, int64_t n_gp,
+                       const eph_ring *rings, int64_t n_ring, const double *t_eph, eph_msystem **out);
+int eph_msystem_load_checkpoint(const char *path, eph_msystem **out);
+int eph_msystem_save_checkpoint(const eph_msystem *h, const char *path);
 int eph_msystem_destroy(eph_msystem *h);
 int64_t eph_msystem_size(const eph_msystem *h);
 int eph_msystem_masses(const eph_msystem *h, eph_mass *out, int64_t cap);
+int eph_msystem_set_masses(eph_msystem *h, const eph_mass *masses, int64_t n);
 int eph_msystem_models(const eph_msystem *h, eph_geopotential *gps, int64_t gcap, eph_ring *rings,
                        int64_t rcap, int64_t *n_gp, int64_t *n_ring);
 int eph_msystem_time(const eph_msystem *h, double *t_eph);
+int eph_msystem_set_time(eph_msystem *h, const double *t_eph);
+/* delta_t, data_cadence, max_ephm_length, combined_delta_t, GM_max_child,
+ * GM_max_parent, GM_max_tiny, Period_max_child (mass.h:214-232) */
+int eph_msystem_params(const eph_msystem *h, double *params8, int64_t *integrator);
 int eph_msystem_accel(eph_msystem *h, int parallel_option);
 int eph_msystem_integrate(eph_msystem *h, double dt, int64_t n_step, int use_gpu);
+int eph_msystem_combined_integrate(eph_msystem *h, double dt, int64_t n_combine, int64_t n_step, int use_gpu);
+int eph_msystem_update(eph_msystem *h, double t);
+int eph_msystem_analyse(eph_msystem *h, int reconstruct, double *t_update);
+int64_t eph_msystem_barycens(const eph_msystem *h, int64_t *ids, double *gm_sys, int64_t *children, int64_t cap,
+                             int64_t child_cap);
+int eph_msystem_scale(eph_msystem *h, int64_t mid, double factor);
+int eph_msystem_scale_geopotential(eph_msystem *h, int64_t mid, double factor);
+int eph_msystem_scale_ring(eph_msystem *h, int64_t mid, double factor);
+int eph_msystem_set_tidal(eph_msystem *h, int64_t parent, const double *matrix9, const int64_t *children,
+                          int64_t n_children);
+/* worker threads of the CPU back end (0 = all hardware threads); results do
+ * not depend on it */
+int eph_msystem_set_threads(eph_msystem *h, int n_threads);
+int64_t eph_msystem_get_mid(const eph_msystem *h, const char *ssid);
 int eph_msystem_diagnostics(const eph_msystem *h, int64_t *body, int *kind, double *value,
                             double *limit, int64_t cap);
 

@@ -1202,7 +1202,7 @@ o_system *o_load_config(const char *config_path, int pool_at_load) {
     char line[1100], key[1100], val[1100];
     char initial[1100] = "", extra[1100] = "", gpd[1100] = "", rgd[1100] = "";
     double delta_t = 300;
-    long long tdb = 0;
+    double tdb = 0;
     while (read_line(f, line, sizeof line)) {
         if (sscanf(line, "%s%s", key, val) != 2) continue;
         if (!strcmp(key, "Initial")) strcpy(initial, val);
@@ -1210,7 +1210,7 @@ o_system *o_load_config(const char *config_path, int pool_at_load) {
         else if (!strcmp(key, "Geopotentials")) strcpy(gpd, val);
         else if (!strcmp(key, "Rings")) strcpy(rgd, val);
         else if (!strcmp(key, "Delta_t")) delta_t = atof(val);
-        else if (!strcmp(key, "TDB")) tdb = atoll(val);
+        else if (!strcmp(key, "TDB")) tdb = atof(val);      /* LOAD_CONFIG, mass_loader.cpp:206-214 */
     }
     fclose(f);
     char p1[2300], p2[2300], p3[2300], p4[2300];
@@ -1219,7 +1219,22 @@ o_system *o_load_config(const char *config_path, int pool_at_load) {
     join_path(p3, sizeof p3, dir, gpd);
     join_path(p4, sizeof p4, dir, rgd);
     o_system *s = o_load_files(p1, extra[0] ? p2 : NULL, gpd[0] ? p3 : NULL, rgd[0] ? p4 : NULL, pool_at_load);
-    if (s) { s->delta_t = delta_t; s->t_eph = dd((double)tdb); }
+    if (!s) return NULL;
+    if (!(fabs(tdb) < 0x1p63) || (double)(int64_t)tdb != tdb) {      /* mass_loader.cpp:226-229 */
+        snprintf(g_err, sizeof g_err, "TDB (initial ephemeris time, seconds) should be an integer");
+        o_free(s);
+        return NULL;
+    }
+    s->delta_t = delta_t;
+    s->t_eph = dd(tdb);
+    for (int64_t i = 0; i < s->n; ++i) {          /* time variables back to t = 0, mass_loader.cpp:231-237 */
+        o_mass *m = &s->m[i];
+        m->GM0 = m->GM - m->dGM * tdb;
+        m->exJ2 = m->dJ2 * tdb;
+        m->C_static.x.x -= m->exJ2 / 2;
+        m->C_static.y.y -= m->exJ2 / 2;
+        m->C_static.z.z += m->exJ2;
+    }
     return s;
 }
 

@@ -1,0 +1,111 @@
+"""The product's CPU-thread back end (msystem::accel(0/-1) and
+integrate(dt, n, USE_GPU=0), src/physics/mass.cpp:216-442 and
+src/integrators/RungeKutta.impl:107-194) and its loader, against the
+reference's own compiled task bodies (golden vectors) and the oracle.  No GPU:
+the product loads and computes on the host.  Bit-exact everywhere, including
+the ring model, since both sides use the host's libm."""
+import numpy as np
+import pytest
+
+from ephemeris_amd.layout import MASS_DTYPE
+from ephemeris_amd.layout import ACCEL_OUTPUTS
+from ephemeris_amd.system import MSystem
+from oracle import pyoracle as po
+from tests.util import CONFIGS, GOLDEN, bits_equal, load_npz
+
+STATE = [n for n in MASS_DTYPE.names if n.split("_")[0] in ("r", "v", "GL", "s", "w")] + ["GM", "exJ2"]
+
+
+def _mismatch(a, b, fields):
+    return {f: int((a[f].view(np.uint64) != b[f].view(np.uint64)).sum()) for f in fields if not bits_equal(a[f], b[f])}
+
+
+@pytest.mark.parametrize("name", ["small25", "full504"])
+def test_load_matches_oracle(name):
+    """Loader: parse, deform, C_static adjustment, sanity, initial accel in the
+    single-thread order (mass_loader.cpp:324-561), all on the host."""
+    m = MSystem(CONFIGS[name]).masses
+    o = po.OSystem.load(CONFIGS[name], pool_at_load=False)
+    ref = o.masses.copy()
+    bad = _mismatch(m, ref, [f for f in MASS_DTYPE.names if not f.startswith(("Egrad", "idaccel", "idtorque",
+                                                                               "jdaccel", "jdtorque"))])
+    assert not bad, bad
+
+
+@pytest.mark.parametrize("name,tag,option", [("small25", "serial", -1), ("full504", "pool", 0),
+                                             ("full504", "serial", -1)])
+def test_accel_matches_reference_bitwise(name, tag, option):
+    d = load_npz(GOLDEN / f"accel_{name}_{tag}.npz")
+    s = MSystem.from_masses(d["masses_in"], d["gps"], d["rings"], threads=4)
+    s.accel(option)
+    bad = _mismatch(s.masses, d["masses_out"], ACCEL_OUTPUTS)
+    assert not bad, bad
+
+
+def test_results_do_not_depend_on_thread_count():
+    d = load_npz(GOLDEN / "accel_full504_pool.npz")
+    outs = []
+    for th in (1, 3, 8):
+        s = MSystem.from_masses(d["masses_in"], d["gps"], d["rings"], threads=th)
+        s.accel(0)
+        outs.append(s.masses)
+    assert bits_equal(outs[0], outs[1]) and bits_equal(outs[0], outs[2])
+
+
+def test_integrate_small25_matches_oracle_trajectory():
+    """288 CPU RK12 steps (1 day) against the oracle trajectory, bit for bit."""
+    s = MSystem(CONFIGS["small25"])
+    s.integrate(300.0, 288, use_gpu=0)
+    ref = load_npz(GOLDEN / "traj_small25.npz")
+    m = s.masses
+    bad = _mismatch(m, ref, [f for f in ref if f in MASS_DTYPE.names])
+    assert not bad, bad
+    assert sum(s.t_eph()) == float(ref["t_eph"])
+
+
+def test_integrate_full504_matches_oracle():
+    """Three CPU RK12 steps of the 504-body system (thread-pool order with
+    the non-point rounds, constant-time-lag tides, ring) against the oracle."""
+    s = MSystem(CONFIGS["full504"], threads=8)
+    o = po.OSystem.load(CONFIGS["full504"], pool_at_load=False)
+    o.set_threads(8)
+    s.integrate(300.0, 3, use_gpu=0)
+    o.integrate(300.0, 3, pool=True)
+    bad = _mismatch(s.masses, o.masses, [f for f in MASS_DTYPE.names if not f.startswith(("idaccel", "idtorque",
+                                                                                          "jdaccel", "jdtorque"))])
+    assert not bad, bad
+
+
+def test_integrate_backwards_returns_close_to_start():
+    s = MSystem(CONFIGS["small25"])
+    r0 = s.masses["r_x_hi"].copy()
+    s.integrate(300.0, 24, use_gpu=0)
+    s.integrate(300.0, -24, use_gpu=0)
+    r1 = s.masses["r_x_hi"]
+    assert np.max(np.abs(r1 - r0) / np.abs(r0)) < 1e-12
+    assert s.t_eph()[0] + s.t_eph()[1] == 0.0
+
+
+def test_timestep_diagnostic_reported():
+    s = MSystem(CONFIGS["small25"])
+    s.integrate(40000.0, 1, use_gpu=0)          # far too large for the close moons
+    assert any(k == "timestep" for _, k, _, _ in s.diagnostics())
+
+
+def test_tidal_matrix_correction_conserves_momentum():
+    """mass.cpp:376-387: the correction adds GM_p dg to each child and
+    -GM_c dg to the parent, so sum GM daccel is unchanged."""
+    d = load_npz(GOLDEN / "accel_small25_serial.npz")
+    a = MSystem.from_masses(d["masses_in"], d["gps"], d["rings"])
+    b = MSystem.from_masses(d["masses_in"], d["gps"], d["rings"])
+    tm = np.array([[2e-13, 1e-14, 0], [1e-14, -1e-13, 3e-15], [0, 3e-15, -1e-13]])
+    b.set_tidal(1, tm, [9, 10])
+    a.accel(-1)
+    b.accel(-1)
+    ma, mb = a.masses, b.masses
+    dda = np.stack([mb[f"daccel_{c}"] - ma[f"daccel_{c}"] for c in "xyz"], -1)
+    assert np.count_nonzero(np.abs(dda).sum(1)) == 3             # parent and two children changed
+    p = (mb["GM"][:, None] * dda).sum(0)
+    assert np.all(np.abs(p) <= 1e-25 + 1e-12 * np.abs(mb["GM"][:, None] * dda).max())
+    for f in ("gaccel_x", "naccel_y", "dtorque_z", "phi"):
+        assert bits_equal(ma[f], mb[f])
