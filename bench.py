@@ -149,7 +149,7 @@ def serving_leg(n_jobs, rate, seed, verify):
     cfg = gen.write_system(f9, tempfile.mkdtemp(prefix="eph_f9_"), "Field9")
 
     def field():
-        m = MSystem(cfg)
+        m = MSystem(cfg, analyse=False)
         gps, rings = m.models()
         f = GpuSystems(m.masses, None, gps, rings, [m.t_eph()])
         m.close()

@@ -58,6 +58,7 @@ SIGNATURES = {
     "eph_server_stats_read": (I, [P, P]),
     "eph_server_destroy": (I, [P]),
     "eph_msystem_load": (I, [C.c_char_p, I, PP]),
+    "eph_msystem_load_ex": (I, [C.c_char_p, I, I, PP]),
     "eph_msystem_create": (I, [I64, P, P, I64, P, I64, P, PP]),
     "eph_msystem_load_checkpoint": (I, [C.c_char_p, PP]),
     "eph_msystem_save_checkpoint": (I, [P, C.c_char_p]),

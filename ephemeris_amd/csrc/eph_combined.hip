@@ -28,7 +28,7 @@ ddm3 DM(const eph_ddmat &a) { return ddm3{DV(a.x), DV(a.y), DV(a.z)}; }
 eph_ddmat E(const ddm3 &a) { return eph_ddmat{E(a.x), E(a.y), E(a.z)}; }
 
 const ddv3 kZero3{dd{0, 0}, dd{0, 0}, dd{0, 0}};
-constexpr int64_t kMaxAnalysePasses = 256;
+constexpr int64_t kMaxAnalysePasses = 64;
 // real(x) != real(x): double-double comparison of dfloat_t.h:42-47, NaN test
 bool dd_isnan(dd a) { return todbl(sub(a, a)) != 0 || a.hi != a.hi; }
 bool dd_eq(dd a, dd b) { return todbl(sub(a, b)) == 0; }

@@ -247,7 +247,7 @@ int MSystem::set_threads(int n) {
 
 int MSystem::threads() const { return pool_ ? pool_->threads() : n_threads; }
 
-int MSystem::load(const std::string &config_path, int summation_at_load) {
+int MSystem::load(const std::string &config_path, int summation_at_load, bool analyse_at_load) {
     LineReader rd(config_path);
     if (!rd.f) { err = config_path + ": file not found"; return fail(EPH_EIO, err); }
     static const char *keys[] = {"Initial", "Extra", "Geopotentials", "Rings", "TDB", "Integrator", "Delta_t",
@@ -271,7 +271,7 @@ int MSystem::load(const std::string &config_path, int summation_at_load) {
     const std::string dir = dir_of(config_path) + "/";
     int rc = load_files(dir + vals[kInitial], vals[kExtra].empty() ? "" : dir + vals[kExtra],
                         vals[kGp].empty() ? "" : dir + vals[kGp], vals[kRings].empty() ? "" : dir + vals[kRings],
-                        summation_at_load);
+                        summation_at_load, analyse_at_load);
     if (rc) return rc;
     if (!vals[kIntegrator].empty()) {                     // mass_loader.cpp:188-204
         static const char *names[] = {"CPU_RK12", "GPU_RK12", "COMBINED_RK12"};
@@ -317,7 +317,7 @@ int MSystem::load(const std::string &config_path, int summation_at_load) {
 }
 
 int MSystem::load_files(const std::string &initial, const std::string &extra, const std::string &gp_dir,
-                        const std::string &ring_dir, int summation_at_load) {
+                        const std::string &ring_dir, int summation_at_load, bool analyse_at_load) {
     mlist.clear();
     gps.clear();
     rgs.clear();
@@ -415,10 +415,10 @@ int MSystem::load_files(const std::string &initial, const std::string &extra, co
             else { err = extra + ": unknown parameter " + sname.data(); return fail(EPH_EIO, err); }
         }
     }
-    return finish_load(summation_at_load);
+    return finish_load(summation_at_load, analyse_at_load);
 }
 
-int MSystem::finish_load(int summation_at_load) {
+int MSystem::finish_load(int summation_at_load, bool analyse_at_load) {
     const CpuSystem cs = cpu_view();
     cpu_deform(cs, pool());                               // msystem::deform, mass_loader.cpp:541
     std::string why;
@@ -438,7 +438,7 @@ int MSystem::finish_load(int summation_at_load) {
     if (!why.empty()) { err = "sanity check failed: " + why; return fail(EPH_ESANITY, err); }
     int rc = cpu_accel_now(summation_at_load == EPH_SUMMATION_SERIAL ? -1 : 0);   // mass_loader.cpp:555
     if (rc) return rc;
-    analyse(true);                                        // mass_loader.cpp:558
+    if (analyse_at_load) analyse(true);                   // mass_loader.cpp:558
     return EPH_OK;
 }
 
@@ -678,13 +678,17 @@ static const MSystem *H(const eph_msystem *h) { return reinterpret_cast<const MS
 
 extern "C" {
 
-int eph_msystem_load(const char *config_path, int summation_at_load, eph_msystem **out) {
+int eph_msystem_load_ex(const char *config_path, int summation_at_load, int analyse, eph_msystem **out) {
     if (!config_path || !out) return eph::fail(EPH_EINVAL, "null argument");
     auto *m = new MSystem();
-    int rc = m->load(config_path, summation_at_load);
+    int rc = m->load(config_path, summation_at_load, analyse != 0);
     if (rc) { delete m; return rc; }
     *out = reinterpret_cast<eph_msystem *>(m);
     return EPH_OK;
+}
+
+int eph_msystem_load(const char *config_path, int summation_at_load, eph_msystem **out) {
+    return eph_msystem_load_ex(config_path, summation_at_load, 1, out);
 }
 
 int eph_msystem_create(int64_t n, const eph_mass *masses, const eph_geopotential *gps, int64_t n_gp,

@@ -52,10 +52,12 @@ public:
     // integration parameters.  The initial force evaluation runs on the CPU
     // back end in the order `summation_at_load` selects (the reference loads
     // from a thread without a pool: EPH_SUMMATION_SERIAL).
-    int load(const std::string &config_path, int summation_at_load = EPH_SUMMATION_SERIAL);
+    // analyse = false skips the orbit-structure analysis of the loader (for
+    // callers that never use the structure: ensemble members, particle fields)
+    int load(const std::string &config_path, int summation_at_load = EPH_SUMMATION_SERIAL, bool analyse = true);
     // msystem::load(fbase, fext, gppath, ringpath) (mass_loader.cpp:324-561)
     int load_files(const std::string &initial, const std::string &extra, const std::string &gp_dir,
-                   const std::string &ring_dir, int summation_at_load = EPH_SUMMATION_SERIAL);
+                   const std::string &ring_dir, int summation_at_load = EPH_SUMMATION_SERIAL, bool analyse = true);
     // a system from bodies and models in memory (model indices into gps/rings)
     int assign(int64_t n, const eph_mass *masses, const eph_geopotential *gps, int64_t n_gp, const eph_ring *rings,
                int64_t n_ring, dd t);
@@ -114,7 +116,7 @@ public:
     const std::string &error() const { return err; }
 
 private:
-    int finish_load(int summation_at_load);
+    int finish_load(int summation_at_load, bool analyse);
     CpuSystem cpu_view();
     int ensure_device();
     int sync_down();

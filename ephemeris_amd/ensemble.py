@@ -31,7 +31,7 @@ def load_members(configs: Sequence[Path]):
     masses, sys_n, gps, rings, teph = [], [], [], [], []
     ngp = nring = 0
     for cfg in configs:
-        m = MSystem(cfg)
+        m = MSystem(cfg, analyse=False)
         mm = m.masses
         g, r = m.models()
         mm["gp"] = np.where(mm["gp"] >= 0, mm["gp"] + ngp, -1)

@@ -31,7 +31,7 @@ class ParticleSet:
 
     @classmethod
     def from_config(cls, field_config, r, v, **kw):
-        m = MSystem(field_config)
+        m = MSystem(field_config, analyse=False)
         gps, rings = m.models()
         field = GpuSystems(m.masses, None, gps, rings, [m.t_eph()])
         m.close()

@@ -120,10 +120,12 @@ class MSystem:
 
     CPU_RK12, GPU_RK12, COMBINED_RK12 = 0, 1, 2
 
-    def __init__(self, config_path=None, summation_at_load=_lib.SUMMATION_SERIAL, *, handle=None, threads=0):
+    def __init__(self, config_path=None, summation_at_load=_lib.SUMMATION_SERIAL, *, handle=None, threads=0,
+                 analyse=True):
         h = C.c_void_p() if handle is None else handle
         if handle is None:
-            _lib.check(_lib.lib().eph_msystem_load(str(config_path).encode(), summation_at_load, C.byref(h)))
+            _lib.check(_lib.lib().eph_msystem_load_ex(str(config_path).encode(), summation_at_load, int(bool(analyse)),
+                                                      C.byref(h)))
         self.h = h
         if threads:
             self.set_threads(threads)
@@ -222,6 +224,11 @@ class MSystem:
         t = (C.c_double * 2)()
         _lib.check(_lib.lib().eph_msystem_analyse(self.h, int(bool(reconstruct)), t))
         return t[0], t[1]
+
+    def analysis_ok(self):
+        """False when the last orbit-structure analysis did not settle."""
+        t = (C.c_double * 2)()
+        return _lib.lib().eph_msystem_analyse(self.h, 0, t) == 0
 
     def barycens(self):
         """[(pid, hid, gid, tid, mid, children, GM_sys)] of the orbit hierarchy."""

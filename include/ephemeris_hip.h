@@ -285,6 +285,11 @@ int eph_profile_reset(eph_system *sys);
  * returns the number of barycens. */
 typedef struct eph_msystem eph_msystem;
 int eph_msystem_load(const char *config_path, int summation_at_load, eph_msystem **out);
+/* as eph_msystem_load; analyse = 0 skips the loader's orbit-structure
+ * analysis (for systems whose structure is never used: ensemble members,
+ * particle fields).  No counterpart in the reference, whose loader always
+ * analyses (mass_loader.cpp:558). */
+int eph_msystem_load_ex(const char *config_path, int summation_at_load, int analyse, eph_msystem **out);
 int eph_msystem_create(int64_t n, const eph_mass *masses, const eph_geopotential *gps, int64_t n_gp,
                        const eph_ring *rings, int64_t n_ring, const double *t_eph, eph_msystem **out);
 int eph_msystem_load_checkpoint(const char *path, eph_msystem **out);

@@ -766,6 +766,70 @@ void o_update(o_system *s, double t) {
     }
 }
 
+/* mass::scale, src/physics/mass.cpp:8-26 */
+void o_scale(o_system *s, int64_t mid, double factor) {
+    o_mass *m = &s->m[mid];
+    double f2 = factor * factor, f3 = factor * f2;
+    m->R *= factor;
+    m->R2 *= f2;
+    m->sR2 *= f2;
+    m->GL = ddvscale(dd(f2), m->GL);
+    m->GI = mscale(f2, m->GI);
+    m->GM *= f3;
+    m->GM0 *= f3;
+    m->dGM *= f3;
+    m->lum *= f2;
+    m->recpt *= factor;
+    m->rR2G_4c *= f3;
+}
+
+/* msystem::scale_geopotential, src/modules/mass_loader.cpp:1008-1021, with
+ * geopotential::copy(gp, factor), src/physics/geopotential.cpp:154-166 */
+void o_scale_geopotential(o_system *s, int64_t mid, double factor) {
+    o_mass *mi = &s->m[mid];
+    if (mi->gp >= 0 && factor != 0) {
+        o_gp g = s->gp[mi->gp];
+        if (factor != 1) {
+            int64_t nt = g.Nz * (g.Nz + 2) - 3;
+            for (int64_t q = 0; q < nt; ++q) g.c[q] *= factor;
+        }
+        s->gp = (o_gp *)realloc(s->gp, sizeof(o_gp) * (size_t)(s->n_gp + 1));
+        s->gp[s->n_gp] = g;
+        mi->gp = s->n_gp++;
+    } else
+        mi->gp = -1;
+    mi->C_potential = mscale(factor, mi->C_potential);
+    mi->C_static = mscale(factor, mi->C_static);
+    mi->k2 *= factor;
+    mi->k2r *= factor;
+    mi->exJ2 *= factor;
+    mi->dJ2 *= factor;
+    mi->GI = mscale(-2.0 / 3 * mi->R2, msubdiag(mi->C_potential, mi->A));
+    mi->GL = ddv(mmulv(mi->GI, mi->w));
+}
+
+/* msystem::scale_ring, src/modules/mass_loader.cpp:1022-1026, with
+ * ring::copy(rp, factor), src/physics/ring.cpp:118-134 */
+void o_scale_ring(o_system *s, int64_t mid, double factor) {
+    o_mass *mi = &s->m[mid];
+    if (mi->ring >= 0 && factor != 0) {
+        o_ring r = s->ring[mi->ring];
+        if (factor != 1) {
+            double host_ratio = 1 - r.GM_ratio;
+            double new_ratio = r.GM_ratio * factor;
+            r.GM_ratio = new_ratio / (host_ratio + new_ratio);
+            r.A *= factor;
+            r.J2 *= factor;
+            r.GL_R2 *= factor;
+            for (int64_t q = 0; q < r.N; ++q) r.GsR2[q] *= factor;
+        }
+        s->ring = (o_ring *)realloc(s->ring, sizeof(o_ring) * (size_t)(s->n_ring + 1));
+        s->ring[s->n_ring] = r;
+        mi->ring = s->n_ring++;
+    } else
+        mi->ring = -1;
+}
+
 /* ------------------------------------------------------------------------- */
 /* RK12, src/integrators/RungeKutta.impl:107-194 (CPU macros :20-35)           */
 /* ------------------------------------------------------------------------- */

@@ -95,6 +95,11 @@ void o_deform(o_system *s);
 void o_integrate(o_system *s, double dt, int64_t n_step, int pool);
 /* msystem::update (src/physics/mass.cpp:28-48) */
 void o_update(o_system *s, double t);
+/* mass::scale (mass.cpp:8-26) and msystem::scale_geopotential / scale_ring
+ * (mass_loader.cpp:1005-1026) */
+void o_scale(o_system *s, int64_t mid, double factor);
+void o_scale_geopotential(o_system *s, int64_t mid, double factor);
+void o_scale_ring(o_system *s, int64_t mid, double factor);
 
 /* --- single functions, exported for golden-vector checks -------------------- */
 void o_dd_ops(const double *a, const double *b, double *out /* 8: +,-,*,/ */);

@@ -82,6 +82,9 @@ def lib():
         L.o_deform.argtypes = [P]
         L.o_integrate.argtypes = [P, C.c_double, I64, I]
         L.o_update.argtypes = [P, C.c_double]
+        L.o_scale.argtypes = [P, I64, C.c_double]
+        L.o_scale_geopotential.argtypes = [P, I64, C.c_double]
+        L.o_scale_ring.argtypes = [P, I64, C.c_double]
         L.o_dd_ops.argtypes = [D, D, D]
         L.o_dd_sqrt.argtypes = [D, D]
         L.o_rotation_matrix.argtypes = [D, C.c_double, D]
@@ -149,6 +152,20 @@ class OSystem:
 
     def integrate(self, dt, n_step, pool=True):
         lib().o_integrate(self.h, float(dt), int(n_step), int(pool))
+
+    def scale(self, mid, factor):
+        lib().o_scale(self.h, int(mid), float(factor))
+
+    def scale_geopotential(self, mid, factor):
+        lib().o_scale_geopotential(self.h, int(mid), float(factor))
+
+    def scale_ring(self, mid, factor):
+        lib().o_scale_ring(self.h, int(mid), float(factor))
+
+    def mid(self, ssid):
+        key = int.from_bytes(ssid.encode().ljust(8, b"\0"), "little")
+        hits = np.nonzero(self.masses["sid"] == key)[0]
+        return int(hits[0]) if len(hits) else -1
 
     def raw_models(self):
         """(gp pointer, n_gp, ring pointer, n_ring) of the o_system."""

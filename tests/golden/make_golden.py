@@ -43,6 +43,8 @@ def ref_lib():
     L.ref_normalize.argtypes = [D]
     L.ref_normalize.restype = C.c_double
     L.ref_asc_node.argtypes = [D, D]
+    L.ref_analyse.restype = I64
+    L.ref_analyse.argtypes = [P, I64, C.c_double, C.c_double, P, I64, P, I64, P]
     return L
 
 
@@ -50,10 +52,17 @@ def dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-def accel_fixtures(L):
-    cases = [("small25", "Small_Config.txt", [1]), ("full504", "Full_Config.txt", [0, 1])]
-    for name, cfg, modes in cases:
-        s = po.OSystem.load(HERE / name / cfg, pool_at_load=False)
+CASES = {"small25": (HERE / "small25" / "Small_Config.txt", [1]),
+         "full504": (HERE / "full504" / "Full_Config.txt", [0, 1]),
+         # the reference's own Solar System (initials/SolarSystem, copied as
+         # data): tesseral geopotentials of the Moon and Mars, Saturn's ring
+         "solar504": (HERE / "solarsystem" / "SolarSystem_Config.txt", [0, 1])}
+
+
+def accel_fixtures(L, names=("small25", "full504")):
+    for name in names:
+        cfg, modes = CASES[name]
+        s = po.OSystem.load(cfg, pool_at_load=False)
         gps, rings = s.models()
         for mode in modes:
             inp = s.masses.copy()
@@ -64,8 +73,10 @@ def accel_fixtures(L):
             tag = "pool" if mode == 0 else "serial"
             np.savez_compressed(HERE / f"accel_{name}_{tag}.npz", masses_in=inp, masses_out=out,
                                 gps=gps, rings=rings)
-        # deform_by of a few bodies
+        # deform_by of a few bodies (and of every body with a model)
         idx = np.arange(0, len(s), max(1, len(s) // 16))
+        if name == "solar504":
+            idx = np.union1d(idx, np.nonzero((s.masses["gp"] >= 0) | (s.masses["ring"] >= 0))[0])
         outs = []
         for i in idx:
             m = np.ascontiguousarray(s.masses.copy())
@@ -147,8 +158,32 @@ def function_fixtures(L):
     np.savez_compressed(HERE / "functions.npz", **out)
 
 
+def analyse_fixtures(L, names):
+    """msystem::analyse(true) of the reference (barycen_analyzer.cpp) on the
+    loaded bodies: the orbit hierarchy and each node's GM_sys."""
+    for name in names:
+        cfg, _ = CASES[name]
+        s = po.OSystem.load(cfg, pool_at_load=False)
+        m = np.ascontiguousarray(s.masses.copy())
+        n = len(m)
+        cap = 2 * n + 16
+        ids = np.zeros((cap, 7), np.int64)
+        ch = np.zeros(4 * cap, np.int64)
+        gm = np.zeros((cap, 2))
+        nb = L.ref_analyse(m.ctypes.data, n, 0.0, 0.0, ids.ctypes.data, cap, ch.ctypes.data, len(ch), gm.ctypes.data)
+        assert 0 < nb <= cap and ids[:nb, 5].sum() <= len(ch)
+        np.savez_compressed(HERE / f"analyse_{name}.npz", masses_in=m, ids=ids[:nb], gm_sys=gm[:nb],
+                            children=ch[:ids[:nb, 5].sum()])
+
+
 if __name__ == "__main__":
     L = ref_lib()
-    accel_fixtures(L)
-    function_fixtures(L)
+    if len(sys.argv) > 1 and sys.argv[1] == "solar":
+        # only the fixtures of the reference's Solar System and the analyses
+        accel_fixtures(L, ["solar504"])
+        analyse_fixtures(L, ["small25", "solar504"])
+    else:
+        accel_fixtures(L)
+        function_fixtures(L)
+        analyse_fixtures(L, ["small25", "solar504"])
     print("fixtures written to", HERE)

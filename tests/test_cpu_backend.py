@@ -20,7 +20,7 @@ def _mismatch(a, b, fields):
     return {f: int((a[f].view(np.uint64) != b[f].view(np.uint64)).sum()) for f in fields if not bits_equal(a[f], b[f])}
 
 
-@pytest.mark.parametrize("name", ["small25", "full504"])
+@pytest.mark.parametrize("name", ["small25", "full504", "solar504"])
 def test_load_matches_oracle(name):
     """Loader: parse, deform, C_static adjustment, sanity, initial accel in the
     single-thread order (mass_loader.cpp:324-561), all on the host."""
@@ -33,7 +33,8 @@ def test_load_matches_oracle(name):
 
 
 @pytest.mark.parametrize("name,tag,option", [("small25", "serial", -1), ("full504", "pool", 0),
-                                             ("full504", "serial", -1)])
+                                             ("full504", "serial", -1), ("solar504", "pool", 0),
+                                             ("solar504", "serial", -1)])
 def test_accel_matches_reference_bitwise(name, tag, option):
     d = load_npz(GOLDEN / f"accel_{name}_{tag}.npz")
     s = MSystem.from_masses(d["masses_in"], d["gps"], d["rings"], threads=4)
@@ -109,3 +110,36 @@ def test_tidal_matrix_correction_conserves_momentum():
     assert np.all(np.abs(p) <= 1e-25 + 1e-12 * np.abs(mb["GM"][:, None] * dda).max())
     for f in ("gaccel_x", "naccel_y", "dtorque_z", "phi"):
         assert bits_equal(ma[f], mb[f])
+
+
+@pytest.mark.parametrize("name", ["small25", "solar504"])
+def test_analyse_matches_reference(name):
+    """msystem::analyse(true) (barycen_analyzer.cpp:12-333): the hierarchy
+    (pid, hid, gid, tid, mid, children) and every node's GM_sys equal the
+    reference's own compiled analysis."""
+    d = load_npz(GOLDEN / f"analyse_{name}.npz")
+    m = d["masses_in"].copy()
+    m["gp"] = m["ring"] = -1                       # the analysis reads only r, v, GM, R and sR2
+    s = MSystem.from_masses(m)
+    s.analyse(True)
+    got = s.barycens()
+    ids, ch, gm = d["ids"], d["children"], d["gm_sys"]
+    ref = [(int(a[0]), int(a[1]), int(a[2]), int(a[3]), int(a[4]), [int(c) for c in ch[a[6]:a[6] + a[5]]],
+            gm[i, 0] + gm[i, 1]) for i, a in enumerate(ids)]
+    assert len(got) == len(ref)
+    bad = [i for i, (a, b) in enumerate(zip(got, ref)) if a[:6] != b[:6] or not bits_equal(np.array(a[6]), np.array(b[6]))]
+    assert not bad, bad[:10]
+
+
+def test_analyse_gives_up_on_a_cycling_hierarchy():
+    """On the 504-body synthetic system three moons' mutual influences tie
+    and the reference's analysis alternates between two hierarchies for ever
+    (its own compiled analysis does not return); here it stops after a
+    bounded number of passes, reports it, and keeps the previous structure."""
+    from ephemeris_amd import _lib
+    s = MSystem(CONFIGS["full504"])
+    assert len(s.barycens()) == 0
+    with pytest.raises(_lib.EphError):
+        s.analyse(True)
+    with pytest.raises(_lib.EphError):
+        s.combined_integrate(300.0, 4, 1, use_gpu=0)

@@ -44,7 +44,7 @@ def test_particles_1e6_sample_equal_alone():
     r, v = gen.make_particles(1000000, field9=f9)
 
     def field():
-        m = MSystem(cfg)
+        m = MSystem(cfg, analyse=False)
         gps, rings = m.models()
         f = GpuSystems(m.masses, None, gps, rings, [m.t_eph()])
         m.close()

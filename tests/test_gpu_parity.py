@@ -55,7 +55,8 @@ def _ring_tolerant_mismatch(m, ref):
     return bad
 
 
-@pytest.mark.parametrize("name,tag", [("small25", "serial"), ("full504", "pool"), ("full504", "serial")])
+@pytest.mark.parametrize("name,tag", [("small25", "serial"), ("full504", "pool"), ("full504", "serial"),
+                                      ("solar504", "pool"), ("solar504", "serial")])
 def test_accel_matches_reference_bitwise(name, tag):
     print(f"\n[case] accel {name} {tag}", flush=True)
     d = load_npz(GOLDEN / f"accel_{name}_{tag}.npz")
@@ -68,9 +69,12 @@ def test_accel_matches_reference_bitwise(name, tag):
     assert not bad
 
 
-def test_deform_matches_reference():
-    print("\n[case] deform full504", flush=True)
-    d = load_npz(GOLDEN / "deform_full504.npz")
+@pytest.mark.parametrize("name", ["full504", "solar504"])
+def test_deform_matches_reference(name):
+    """deform_by of the reference's own code; solar504 is the reference's
+    Solar System, whose Moon and Mars carry tesseral geopotentials."""
+    print(f"\n[case] deform {name}", flush=True)
+    d = load_npz(GOLDEN / f"deform_{name}.npz")
     g = GpuSystems(d["masses_in"], None, d["gps"], d["rings"])
     g.deform()
     m, _ = g.download()
@@ -127,7 +131,7 @@ def test_integrate_is_deterministic_and_chunk_consistent():
     assert bits_equal(ma, mb)
 
 
-@pytest.mark.parametrize("name,tag", [("full504", "pool"), ("small25", "serial")])
+@pytest.mark.parametrize("name,tag", [("full504", "pool"), ("small25", "serial"), ("solar504", "pool")])
 def test_kernel_variants_bitwise(name, tag):
     """Every kernel variant gives the same bits; all match the reference
     (ring hosts to an ulp of their transcendentals)."""

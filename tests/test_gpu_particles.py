@@ -69,7 +69,7 @@ def test_particles_independent_of_batch_composition(field9):
 def test_field_small_path_matches_generic(field9):
     """The persistent one-workgroup path and the per-stage kernels agree."""
     f9, cfg = field9
-    m = MSystem(cfg)
+    m = MSystem(cfg, analyse=False)
     m.integrate(DT, 30)
     a = m.masses
     o = po.OSystem.load(cfg, pool_at_load=False)

@@ -30,7 +30,7 @@ def setup():
 
 
 def _field(cfg):
-    m = MSystem(cfg)
+    m = MSystem(cfg, analyse=False)
     gps, rings = m.models()
     f = GpuSystems(m.masses, None, gps, rings, [m.t_eph()])
     m.close()

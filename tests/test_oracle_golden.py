@@ -64,10 +64,14 @@ def test_ring_sum(which):
         assert bits_equal(o, f[which + "_out"][i]), i
 
 
-@pytest.mark.parametrize("name", ["small25", "full504"])
+CONFIG = {"small25": GOLDEN / "small25" / "Small_Config.txt", "full504": GOLDEN / "full504" / "Full_Config.txt",
+          "solar504": GOLDEN / "solarsystem" / "SolarSystem_Config.txt"}
+
+
+@pytest.mark.parametrize("name", ["small25", "full504", "solar504"])
 def test_deform_by(name):
     d = load_npz(GOLDEN / f"deform_{name}.npz")
-    s = po.OSystem.load(GOLDEN / name / ("Small_Config.txt" if name == "small25" else "Full_Config.txt"))
+    s = po.OSystem.load(CONFIG[name])
     for k, i in enumerate(d["idx"]):
         m = s.masses
         m[:] = d["masses_in"]
@@ -76,10 +80,11 @@ def test_deform_by(name):
             assert bits_equal(np.array([m[fld][i]]), np.array([d["out"][fld][k]])), (i, fld)
 
 
-@pytest.mark.parametrize("name,tag", [("small25", "serial"), ("full504", "pool"), ("full504", "serial")])
+@pytest.mark.parametrize("name,tag", [("small25", "serial"), ("full504", "pool"), ("full504", "serial"),
+                                      ("solar504", "pool"), ("solar504", "serial")])
 def test_accel_matches_reference(name, tag):
     d = load_npz(GOLDEN / f"accel_{name}_{tag}.npz")
-    s = po.OSystem.load(GOLDEN / name / ("Small_Config.txt" if name == "small25" else "Full_Config.txt"))
+    s = po.OSystem.load(CONFIG[name])
     m = s.masses
     m[:] = d["masses_in"]
     s.accel(0 if tag == "pool" else -1, pool=True)

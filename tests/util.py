@@ -6,7 +6,8 @@ import numpy as np
 ROOT = Path(__file__).resolve().parents[1]
 GOLDEN = ROOT / "tests" / "golden"
 CONFIGS = {"small25": GOLDEN / "small25" / "Small_Config.txt",
-           "full504": GOLDEN / "full504" / "Full_Config.txt"}
+           "full504": GOLDEN / "full504" / "Full_Config.txt",
+           "solar504": GOLDEN / "solarsystem" / "SolarSystem_Config.txt"}
 
 
 def load_npz(path):
