@@ -31,6 +31,7 @@ SIGNATURES = {
     "eph_accel": (I, [P, I]),
     "eph_integrate": (I, [P, D, I64]),
     "eph_integrate_ex": (I, [P, D, I64, I]),
+    "eph_accel_noalloc": (I, [P, I64, P, I64, P, I64, I, P]),
     "eph_profile_enable": (I, [P, I]),
     "eph_profile_read": (I, [P, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.POINTER(C.c_int64), I]),
     "eph_profile_reset": (I, [P]),

@@ -8,7 +8,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OUT = HERE / "lib" / "libephemeris_hip.so"
-SOURCES = ["eph_kernels.hip", "eph_runtime.hip", "eph_msystem.hip", "eph_server.hip", "eph_cpu.hip", "eph_combined.hip"]
+SOURCES = ["eph_kernels.hip", "eph_runtime.hip", "eph_msystem.hip", "eph_server.hip", "eph_cpu.hip", "eph_combined.hip", "eph_noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -ffp-contract=off: every multiply and add rounds separately, as in the
 # reference build; packed FP32 instructions are disabled.

@@ -37,7 +37,6 @@ static dd host_add(dd x, dd y) { return add(x, y); }
 
 // ---- eph_mass <-> field-major transposition ----------------------------------
 namespace {
-struct FieldMap { int field; size_t offset; };
 #define EM(member) offsetof(eph_mass, member)
 std::vector<FieldMap> mass_fields() {
     std::vector<FieldMap> m;
@@ -65,8 +64,9 @@ const std::vector<FieldMap> &fields() {
     static const std::vector<FieldMap> f = mass_fields();
     return f;
 }
-constexpr int kMassFieldEnd = F_X0R;   // fields [0, F_X0R) carry eph_mass data
 }  // namespace
+
+const std::vector<FieldMap> &mass_field_map() { return fields(); }
 
 // ---- profiling --------------------------------------------------------------
 const char *const kKernelNames[kNumKernelClasses] = {
@@ -983,7 +983,7 @@ int eph_init(int device) {
     if (strncmp(p.gcnArchName, "gfx950", 6) != 0)
         return fail(EPH_ENODEV, std::string("library built for gfx950, device is ") + p.gcnArchName);
     if (hipSetDevice(device) != hipSuccess) return fail(EPH_ENODEV, "hipSetDevice failed");
-    return EPH_OK;
+    return eph::noalloc_init_device(device);
 }
 
 int eph_system_create(int64_t n_sys, const int64_t *sys_n, const eph_mass *masses,

@@ -151,6 +151,14 @@ int create_particles(Batch *field, int64_t n, const double *r, const double *v, 
 int integrate_particles(Particles *ps, double dt, int64_t n_step);
 int download_particles(Particles *ps, double *r, double *v, double *extrema);
 
+// eph_mass member of each field of the field-major layout, fields
+// [0, kMassFieldEnd) except phi / c^2 and naccel / c^2
+struct FieldMap { int field; size_t offset; };
+constexpr int kMassFieldEnd = F_X0R;   // fields [0, F_X0R) carry eph_mass data
+const std::vector<FieldMap> &mass_field_map();
+// set-up of the allocation-free entry point on the current device
+int noalloc_init_device(int device);
+
 int create_batch(int64_t n_sys, const int64_t *sys_n, const eph_mass *masses,
                  const eph_geopotential *gps, int64_t n_gp, const eph_ring *rings, int64_t n_ring,
                  const double *t_eph, Batch **out);

@@ -124,6 +124,23 @@ int eph_integrate(eph_system *sys, double dt, int64_t n_step);
 /* As eph_integrate but with an explicit summation order for every accel. */
 int eph_integrate_ex(eph_system *sys, double dt, int64_t n_step, int summation);
 
+/* Allocation-free force evaluation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  One msystem::accel
+ * (parallel_option 0 with summation EPH_SUMMATION_POOL, -1 with
+ * EPH_SUMMATION_SERIAL) of the n bodies at `masses` with their models, all in
+ * device memory, written to `out` (device memory, n bodies; may be `masses`);
+ * results are bit-identical to eph_system_create + eph_accel +
+ * eph_system_download.  It calls no hipMalloc or hipFree: the working state
+ * lives in a fixed array of the library on each device (about 2.8 MB, set up
+ * by eph_init or by the first call on a device), shared by every call on that
+ * device, so calls that overlap on one device race.  Limits: n <= 1024
+ * bodies and at most 64 geopotential and ring models in use, else EPH_EINVAL.
+ * The call runs on the null stream and returns when the results are in `out`;
+ * inputs must be ready. */
+int eph_accel_noalloc(const eph_mass *masses, int64_t n, const eph_geopotential *gps, int64_t n_gp,
+                      const eph_ring *rings, int64_t n_ring, int summation, eph_mass *out);
+
 /* ---- tuning ------------------------------------------------------------------
  * Kernel variant of the partner-loop passes: 0 automatic, 1 one thread per
  * body (throughput; large batches), 2 one wavefront per body (latency; few
