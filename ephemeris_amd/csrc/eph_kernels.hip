@@ -1818,8 +1818,16 @@ __device__ __forceinline__ v3 particle_accel(const JMain *__restrict__ T, int nf
 }
 
 // Particle fields (stride pv.npp): r (6), v (6), last acceleration (3),
-// min_distance, max_influence.
-enum PField : int { P_R = 0, P_V = 6, P_A = 12, P_MIND = 15, P_MAXI = 16, P_COUNT = 17 };
+// min_distance, max_influence; then, per lane of the launch, the derivatives
+// of stages kPartLdsStages..24 of the step in progress (3 rows each).
+enum PField : int { P_R = 0, P_V = 6, P_A = 12, P_MIND = 15, P_MAXI = 16, P_COUNT = 17, P_STAGE = 17 };
+// Stage derivatives 0..12 of a step live in LDS (312 B per particle: two
+// 256-particle workgroups per CU), 13..24 in the particle's scratch rows
+// (L2-resident); every access names its address space, so no access goes
+// through a generic pointer.
+constexpr int kPartLdsStages = 13;
+typedef __attribute__((address_space(3))) double lds_f64;
+typedef __attribute__((address_space(1))) double glb_f64;
 
 // n_steps > 0: advance; n_steps < 0: compute the first acceleration from
 // the records; 0: leave the workgroup's particles alone.
@@ -1853,12 +1861,45 @@ __global__ __launch_bounds__(kPartBS) void k_particles(PView pv, int n_steps, do
         return;
     }
     // stage derivatives a_j (j = 0..24) of this step, per particle
-    __shared__ double fa[25 * 3 * kPartBS];
-    auto FA = [&](int j, int c) -> double & { return fa[(j * 3 + c) * kPartBS + threadIdx.x]; };
+    __shared__ double fa[kPartLdsStages * 3 * kPartBS];
+    lds_f64 *fl = (lds_f64 *)fa + threadIdx.x;
+    glb_f64 *fg = (glb_f64 *)(P + (int64_t)P_STAGE * S + p);   // this lane's own column (p < npp)
+    auto get_l = [&](int j) { return v3{fl[(j * 3 + 0) * kPartBS], fl[(j * 3 + 1) * kPartBS], fl[(j * 3 + 2) * kPartBS]}; };
+    auto get_g = [&](int j) {
+        const int64_t r = (int64_t)(j - kPartLdsStages) * 3;
+        return v3{fg[(r + 0) * S], fg[(r + 1) * S], fg[(r + 2) * S]};
+    };
+    auto put = [&](int j, v3 a) {
+        if (j < kPartLdsStages) {
+            fl[(j * 3 + 0) * kPartBS] = a.x; fl[(j * 3 + 1) * kPartBS] = a.y; fl[(j * 3 + 2) * kPartBS] = a.z;
+        } else {
+            const int64_t r = (int64_t)(j - kPartLdsStages) * 3;
+            fg[(r + 0) * S] = a.x; fg[(r + 1) * S] = a.y; fg[(r + 2) * S] = a.z;
+        }
+    };
+    // sum over the non-zero columns l of row j of (a_jl) * a_l, ascending l:
+    // the LDS columns first, then the scratch-row columns
+    auto row_sum = [&](int j) {
+        v3 caj{0, 0, 0};
+        const int bj = j * (j - 1) / 2;
+        const int f1 = c_row0[j + 1];
+        int f = c_row0[j];
+        for (; f < f1 && c_nzj[f] < kPartLdsStages; ++f) {
+            const int l = c_nzj[f];
+            const double2 cl = c_rk[bj + l];
+            caj += (cl.x + cl.y) * get_l(l);
+        }
+        for (; f < f1; ++f) {
+            const int l = c_nzj[f];
+            const double2 cl = c_rk[bj + l];
+            caj += (cl.x + cl.y) * get_g(l);
+        }
+        return caj;
+    };
     v3 a{ld(P_A + 0), ld(P_A + 1), ld(P_A + 2)};
     for (int step = 0; step < n_steps; ++step) {
         const ddv3 r0 = r, v0 = v;
-        FA(0, 0) = a.x; FA(0, 1) = a.y; FA(0, 2) = a.z;
+        put(0, a);
         for (int k = 1; k <= 25; ++k) {
             // x_k = x0 + dt * sum_j a_kj f_j (RungeKutta.impl:148-170); the
             // velocity of stage j is v0 + dt * (sum_l a_jl a_l), recomputed
@@ -1867,32 +1908,31 @@ __global__ __launch_bounds__(kPartBS) void k_particles(PView pv, int n_steps, do
             v3 ca{0, 0, 0};
             const int base = k * (k - 1) / 2;
             const int e1 = c_row0[k + 1];
-            for (int e = c_row0[k]; e < e1; ++e) {             // non-zero columns, ascending
+            int e = c_row0[k];
+            for (; e < e1 && c_nzj[e] < kPartLdsStages; ++e) {   // non-zero columns, ascending
                 const int j = c_nzj[e];
                 const double2 c = c_rk[base + j];
                 const dd ckj{c.x, c.y};
                 const double fc = c.x + c.y;
                 ddv3 vj = v0;
-                if (j > 0) {
-                    v3 caj{0, 0, 0};
-                    const int bj = j * (j - 1) / 2;
-                    const int f1 = c_row0[j + 1];
-#pragma unroll 4
-                    for (int f = c_row0[j]; f < f1; ++f) {
-                        const int l = c_nzj[f];
-                        const double2 cl = c_rk[bj + l];
-                        caj += (cl.x + cl.y) * v3{FA(l, 0), FA(l, 1), FA(l, 2)};
-                    }
-                    vj = add(v0, mkddv(dt * caj));
-                }
+                if (j > 0) vj = add(v0, mkddv(dt * row_sum(j)));
                 cv = add(cv, scale(ckj, vj));
-                ca += fc * v3{FA(j, 0), FA(j, 1), FA(j, 2)};
+                ca += fc * get_l(j);
+            }
+            for (; e < e1; ++e) {
+                const int j = c_nzj[e];
+                const double2 c = c_rk[base + j];
+                const dd ckj{c.x, c.y};
+                const double fc = c.x + c.y;
+                const ddv3 vj = add(v0, mkddv(dt * row_sum(j)));
+                cv = add(cv, scale(ckj, vj));
+                ca += fc * get_g(j);
             }
             r = add(r0, scale(mkdd(dt), cv));
             v = add(v0, mkddv(dt * ca));
             const JMain *T = pv.rec + ((entry0 + 25 * (int64_t)step + (k - 1)) * pv.nrec_sys) * kSmallMax;
             a = particle_accel(T, pv.nf, r, v, pv.sR2, mind, maxi);
-            if (k < 25) { FA(k, 0) = a.x; FA(k, 1) = a.y; FA(k, 2) = a.z; }
+            if (k < 25) put(k, a);
         }
     }
     if (!valid) return;

@@ -15,7 +15,7 @@ constexpr int kSmallMax = 64;    // largest system integrated by one workgroup
 // sum of the non-zero tableau weights of each stage (dt_k), index 1..25
 struct DtK { double v[26]; };
 
-constexpr int kPartBS = 256;     // particles per workgroup (stage storage in LDS: 600 B each)
+constexpr int kPartBS = 256;     // particles per workgroup (stage derivatives in LDS: 312 B each)
 
 // test particles in the recorded field of one small system
 struct PView {

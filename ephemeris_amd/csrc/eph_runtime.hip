@@ -873,7 +873,8 @@ int create_particles(Batch *field, int64_t n, const double *r, const double *v, 
             host[(size_t)(6 + 2 * c) * ps->npp + i] = v[3 * i + c];
         }
     auto bail = [&](int code) { delete ps; return code; };
-    if (hipMalloc(&ps->P, sizeof(double) * host.size()) != hipSuccess) return bail(fail(EPH_ENOMEM, "particle state allocation failed"));
+    if (hipMalloc(&ps->P, sizeof(double) * P_ALLOC_ROWS * ps->npp) != hipSuccess)
+        return bail(fail(EPH_ENOMEM, "particle state allocation failed"));
     if (hipMemcpy(ps->P, host.data(), sizeof(double) * host.size(), hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail(EPH_EDEVICE, "particle upload failed"));
     ps->rec_steps = kRecChunk;

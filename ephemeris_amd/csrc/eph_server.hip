@@ -219,7 +219,7 @@ int Server::admit(int kind, std::vector<SJob *> &fresh) {
             used_slots += is_ens ? (int64_t)it->second->sys_n.size() : it->second->units;
     }
     const int64_t max_slots = is_ens ? cfg.max_systems : cfg.max_particles;
-    const int64_t resident = is_ens ? (int64_t)F_COUNT * 8 * (ens ? ens->nb : 0) : (int64_t)P_COUNT_HOST * 8 * npp;
+    const int64_t resident = is_ens ? (int64_t)F_COUNT * 8 * (ens ? ens->nb : 0) : (int64_t)P_ALLOC_ROWS * 8 * npp;
     int64_t reserve = 0;                   // a re-formed batch holds old and new copies at once
     for (auto qi = queue.begin(); qi != queue.end();) {
         SJob *j = jobs[*qi].get();
@@ -314,7 +314,7 @@ int Server::rebuild_pool(const std::vector<SJob *> &new_part) {
     for (SJob *j : order) npp_new += pad(j->units);
     double *Pn = nullptr;
     if (npp_new) {
-        HIP_CHECK(hipMalloc(&Pn, sizeof(double) * P_COUNT_HOST * npp_new));
+        HIP_CHECK(hipMalloc(&Pn, sizeof(double) * P_ALLOC_ROWS * npp_new));
         int64_t c = 0;
         std::vector<std::vector<double>> staging;
         for (SJob *j : order) {
@@ -610,7 +610,7 @@ int eph_server_submit_particles(eph_server *srv, int64_t n, const double *r, con
     j->units = n;
     j->r.assign(r, r + 3 * n);
     j->v.assign(v, v + 3 * n);
-    j->bytes = (int64_t)eph::P_COUNT_HOST * 8 * ((n + eph::kPartBS - 1) / eph::kPartBS * eph::kPartBS);
+    j->bytes = (int64_t)eph::P_ALLOC_ROWS * 8 * ((n + eph::kPartBS - 1) / eph::kPartBS * eph::kPartBS);
     return s->submit(std::move(j), job);
 }
 
