@@ -291,7 +291,9 @@ constexpr int kDefK4Pitch = 13;
 constexpr int kJDefF = 8;
 __constant__ int c_jdef_field[kJDefF] = {F_R + 0, F_R + 1, F_R + 2, F_R + 3, F_R + 4, F_R + 5, F_SR2, F_GM};
 
-__global__ __launch_bounds__(64) void k_deform_k4(View d, const int4 *__restrict__ blk, int prepare, NpTail np) {
+// Three wavefronts per SIMD (168 registers; the few values the compiler then
+// keeps in scratch are stored before the partner loop and reloaded after it).
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_deform_k4(View d, const int4 *__restrict__ blk, int prepare, NpTail np) {
     if (np_tail(d, np)) return;
     const Acc A{d.F, d.nbp};
     const int4 bk = blk[blockIdx.x];
