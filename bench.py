@@ -35,6 +35,42 @@ DT = 300.0
 # ordered pair, counted from the kernel source (eph_kernels.hip)
 FLOP_MAIN_PAIR = 294          # RELATIVITY 109, tidal (both quadrupoles) 92, Lense-Thirring 83, radiation 10
 FLOP_DEFORM_PAIR = {"nk2": 54, "ndelay": 89, "damped": 217}
+# double-double operations of eph_math.hpp: add/sub 11, mul 9, diff (dd
+# difference rounded to double, per component) 12
+DD_ADD, DD_MUL, DIFF3 = 11, 9, 36
+
+
+def rk12_nonzero_rows():
+    """Non-zero columns of each row k = 1..25 of the RK12 tableau."""
+    import re
+    rows = {}
+    for line in (ROOT / "ephemeris_amd" / "tables" / "rk12_feagin.inc").read_text().splitlines():
+        m = re.match(r"/\* k=\s*(\d+) \*/ (.*)", line)
+        if m:
+            ents = re.findall(r"\{([^,]+),([^}]+)\}", m.group(2))
+            rows[int(m.group(1))] = [j for j, (a, b) in enumerate(ents)
+                                     if float.fromhex(a.strip()) != 0 or float.fromhex(b.strip()) != 0]
+    return rows
+
+
+def particle_flop_per_step(nf, n_quad, n_spin):
+    """Algorithmic FP64 operations of one RK12 step of one particle, counted
+    from k_particles / particle_accel: per stage, the field bodies' potential
+    and Newtonian pull (54 each), the reactions of the quadrupoles (88 each),
+    RELATIVITY (108 each) and the partners' Lense-Thirring terms (31 each),
+    plus 25 fixed; and the stage combination, which re-forms each stage
+    velocity it uses from the stored stage derivatives."""
+    per_stage = nf * 54 + 16 + n_quad * 88 + nf * 108 + n_spin * 31 + 9
+    rows = rk12_nonzero_rows()
+    comb = 0
+    for k in range(1, 26):
+        for j in rows[k]:
+            comb += 1 + 6                                          # fc, ca += fc * a_j
+            comb += 3 * DD_MUL + 3 * DD_ADD                        # cv += c_kj * v_j (dd)
+            if j > 0:
+                comb += 7 * len(rows[j]) + 3 + 3 * DD_ADD          # v_j = v0 + dt * sum a_jl a_l
+        comb += (3 * DD_MUL + 3 * DD_ADD) + (3 + 3 * DD_ADD)      # r_k, v_k
+    return 25 * per_stage + comb
 
 
 PROFILE = ROOT / "profiles" / "kernels_full504_ensemble256_particles1e6.md"
@@ -102,12 +138,33 @@ def batched_leg(steps, warmup, n_clones):
     pairs = ens.pairs_per_eval()
     main_ms = prof["main_force_pool"][0]
     achieved = pairs * FLOP_MAIN_PAIR / (main_ms * 1e-3) / 1e12
+    # deformation pairs by tidal class (mass.impl:1-46): k2 = 0, k2 with no
+    # time lag, constant time lag (one spin pass counted; those bodies take
+    # 2-4, so the deform figure is a lower bound)
+    m, _ = ens.download()
+    cls_pairs = {"nk2": 0, "ndelay": 0, "damped": 0}
+    for k in range(ens.n_members):
+        mk = ens.member(m, k)
+        nm = len(mk)
+        nk2 = int((mk["k2"] == 0).sum())
+        nd = int(((mk["k2"] != 0) & (mk["tide_delay"] != 0)).sum())
+        cls_pairs["nk2"] += nk2 * (nm - 1)
+        cls_pairs["damped"] += nd * (nm - 1)
+        cls_pairs["ndelay"] += (nm - nk2 - nd) * (nm - 1)
+    deform_ms = prof["deform_spin"][0]
+    deform_flop = sum(cls_pairs[c] * FLOP_DEFORM_PAIR[c] for c in cls_pairs)
+    deform_tf = deform_flop / (deform_ms * 1e-3) / 1e12
     return {"members": ens.n_members, "bodies": int(ens.sys_n.sum()), "pairs_per_eval": pairs,
             "steps": steps, "pair_interactions_per_s": steps * 25 * pairs / el,
             "rk12_member_steps_per_s": steps * ens.n_members / el, "ms_per_step": el / steps * 1e3,
             "load_s": t_load, "kernel_variant": {2: "one wavefront per body", 4: "four lanes per body"}.get(variant, variant),
             "roofline_main_force": {"achieved": achieved, "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
                                     "frac": achieved / FP64_VECTOR_PEAK_TFLOPS, "avg_launch_ms": main_ms},
+            "roofline_deform": {"pairs_by_class": cls_pairs, "flop_per_pair": FLOP_DEFORM_PAIR,
+                                "flop_first_pass": deform_flop, "ms_per_evaluation": deform_ms,
+                                "achieved_lower_bound": deform_tf, "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                "frac_lower_bound": deform_tf / FP64_VECTOR_PEAK_TFLOPS,
+                                "note": "both deform launches; constant-time-lag bodies counted for one spin pass"},
             "kernel_ms": {k: round(v[0], 5) for k, v in prof.items()}}
 
 
@@ -130,10 +187,21 @@ def particles_leg(n_particles, steps):
     prof = g.profile_read()
     g.profile(False)
     nf = len(f9.bodies)
+    fm, _ = g.download()
+    n_quad = int((np.stack([fm[f"C_potential_{a}{b}"] for a in "xyz" for b in "xyz"], -1) != 0).any(-1).sum())
+    n_spin = int((np.stack([fm[f"GL_{c}_hi"] for c in "xyz"], -1) != 0).any(-1).sum())
+    flop_step = particle_flop_per_step(nf, n_quad, n_spin)
+    part_ms = prof["particle_steps"][0] * prof["particle_steps"][1] / steps if "particle_steps" in prof else None
+    roof = None
+    if part_ms:
+        tf = n_particles * flop_step / (part_ms * 1e-3) / 1e12
+        roof = {"kernel": "k_particles", "flop_per_particle_step": flop_step, "quadrupole_field_bodies": n_quad,
+                "spinning_field_bodies": n_spin, "ms_per_step": part_ms, "achieved": tf,
+                "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tf / FP64_VECTOR_PEAK_TFLOPS}
     return {"particles": n_particles, "field_bodies": nf, "steps": steps,
             "particle_steps_per_s": n_particles * steps / el,
             "pair_interactions_per_s": n_particles * steps * 25 * nf / el, "ms_per_step": el / steps * 1e3,
-            "kernel_ms": {k: round(v[0], 4) for k, v in prof.items() if v[1]}}
+            "kernel_ms": {k: round(v[0], 4) for k, v in prof.items() if v[1]}, "roofline": roof}
 
 
 def serving_leg(n_jobs, rate, seed, verify):
