@@ -451,9 +451,10 @@ int Batch::step_graph(double dt, int mode) {
 
 // ---- autotuner ------------------------------------------------------------------
 // For a batch in automatic mode with enough thread-pool-order bodies, time
-// one wavefront per body against four lanes per body on a device copy of the
-// batch (results are bit-identical, so only speed differs) and keep the
-// faster; choices are cached per shape bucket for the life of the process.
+// one thread per body, one wavefront per body and four lanes per body on a
+// device copy of the batch (results are bit-identical, so only speed
+// differs) and keep the fastest; choices are cached per shape bucket for the
+// life of the process.
 namespace {
 constexpr int kTuneMinBodies = 2048;
 std::mutex g_tune_mu;
@@ -476,7 +477,7 @@ int Batch::autotune(double dt, int mode) {
     for (int64_t k = 0; k < n_sys; ++k) all[k] = (int)k;
     int best = 0;
     double best_s = 1e300;
-    for (int cand : {2, 4}) {
+    for (int cand : {1, 2, 4}) {
         Batch *c = nullptr;
         if (regroup_batch(this, all, 0, nullptr, nullptr, nullptr, 0, nullptr, 0, nullptr, &c) || !c) {
             g_last_error.clear();                          // not enough memory for a copy: keep the heuristic

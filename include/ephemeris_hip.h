@@ -154,8 +154,8 @@ int eph_accel_noalloc(const eph_mass *masses, int64_t n, const eph_geopotential 
  * src/integrators/RungeKutta.cu:13-18, which changes results). */
 int eph_system_set_variant(eph_system *sys, int variant, int allow_persistent);
 /* Autotuner (default on): in automatic mode, the first eph_integrate of a
- * batch with >= 2048 thread-pool-order bodies times variants 2 and 4 for a
- * few steps on a device copy of the batch and keeps the faster (the choice
+ * batch with >= 2048 thread-pool-order bodies times variants 1, 2 and 4 for
+ * a few steps on a device copy of the batch and keeps the fastest (the choice
  * is cached per batch shape for the process).  Results do not depend on it.
  * eph_system_kernel_variant returns the variant in use (0: heuristic). */
 int eph_system_set_autotune(eph_system *sys, int enable);
