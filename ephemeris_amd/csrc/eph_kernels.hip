@@ -1276,8 +1276,14 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 1 ?
 // while visiting p (ROTATIONAL_TIDAL_DEFORMATION and APPLY_NONPOINT_FORCE)
 // are added before i's own partner loop when p < i and after it when p > i.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void serial_reaction(const View &d, const Acc &A, int gp_, int gi, const JMain &T,
-                                                const IMain &I, v3 &da) {
+// field of a model row (View::san) at partner jl
+__device__ __forceinline__ v3 san_at(const View &d, int row, int jl) {
+    const double *a = d.san + 3 * ((int64_t)row + jl);
+    return v3{a[0], a[1], a[2]};
+}
+
+__device__ __forceinline__ void serial_reaction(const View &d, const Acc &A, int gp_, int gi, int il, const JMain &T,
+                                                const IMain &I, v3 &da, bool rows) {
     // outer body p visiting partner i: r = r_i - r_p
     const ddv3 rp{dd{T.rxh, T.rxl}, dd{T.ryh, T.ryl}, dd{T.rzh, T.rzl}};
     PairGeo p;
@@ -1289,8 +1295,10 @@ __device__ __forceinline__ void serial_reaction(const View &d, const Acc &A, int
     const v3 dg = (p.rr3 * p.rr2 * T.R2) * quad_field(Cp, p);
     da -= T.GM * dg;
     const int gpi = d.np_gp[gp_], rgi = d.np_ring[gp_];
-    if (gpi >= 0) da += T.GM * nonpoint_field_call(d, A, gp_, 0, gpi, p.r);
-    if (rgi >= 0) da += T.GM * nonpoint_field_call(d, A, gp_, 1, rgi, p.r);
+    // the field of p's models at i (r = r_i - r_p): from the rows formed
+    // beforehand (rows), or evaluated here
+    if (gpi >= 0) da += T.GM * (rows ? san_at(d, d.srow_gp[gp_], il) : nonpoint_field_call(d, A, gp_, 0, gpi, p.r));
+    if (rgi >= 0) da += T.GM * (rows ? san_at(d, d.srow_ring[gp_], il) : nonpoint_field_call(d, A, gp_, 1, rgi, p.r));
 }
 
 template <int BS>
@@ -1329,17 +1337,17 @@ __global__ __launch_bounds__(BS) void k_main_serial(View d, const int2 *__restri
                     dq += T.GM * cross(p.r, dg);
                     lt_rad(I, T, p, da, dq);
                     if (my_gp >= 0) {                      // APPLY_NONPOINT_FORCE, mass.impl:127-130
-                        const v3 an = nonpoint_field_call(d, A, gi, 0, my_gp, p.r);
+                        const v3 an = d.san ? san_at(d, d.srow_gp[gi], jl) : nonpoint_field_call(d, A, gi, 0, my_gp, p.r);
                         da -= T.GM * an;
                         dq -= T.GM * cross(p.r, an);
                     }
                     if (my_ring >= 0) {
-                        const v3 an = nonpoint_field_call(d, A, gi, 1, my_ring, p.r);
+                        const v3 an = d.san ? san_at(d, d.srow_ring[gi], jl) : nonpoint_field_call(d, A, gi, 1, my_ring, p.r);
                         da -= T.GM * an;
                         dq -= T.GM * cross(p.r, an);
                     }
                 } else if ((phase == 0) == (jl < il)) {
-                    serial_reaction(d, A, off + jl, gi, T, I, da);
+                    serial_reaction(d, A, off + jl, gi, il, T, I, da, d.san != nullptr);
                 }
             }
         }
@@ -1683,7 +1691,7 @@ __device__ void main_serial_body_tile(const View &d, const Acc &A, int gi, int i
     const int my_gp = d.np_gp[gi], my_ring = d.np_ring[gi];
     v3 ga{0, 0, 0}, da{0, 0, 0}, dq{0, 0, 0};
     double mind = A.ld(F_MIND, gi), maxi = A.ld(F_MAXI, gi);
-    for (int q = 0; q < il; ++q) serial_reaction(d, A, off + q, gi, tile[q], I, da);
+    for (int q = 0; q < il; ++q) serial_reaction(d, A, off + q, gi, il, tile[q], I, da, false);
     for (int q = 0; q < n; ++q) {
         if (q == il) continue;
         const JMain &T = tile[q];
@@ -1705,7 +1713,7 @@ __device__ void main_serial_body_tile(const View &d, const Acc &A, int gi, int i
             dq -= T.GM * cross(p.r, an);
         }
     }
-    for (int q = il + 1; q < n; ++q) serial_reaction(d, A, off + q, gi, tile[q], I, da);
+    for (int q = il + 1; q < n; ++q) serial_reaction(d, A, off + q, gi, il, tile[q], I, da, false);
     A.stv(F_GACC, gi, ga);
     A.st(F_MIND, gi, mind);
     A.st(F_MAXI, gi, maxi);

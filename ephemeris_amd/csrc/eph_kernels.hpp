@@ -43,6 +43,11 @@ struct View {
     double *an;                  // non-point field per (item, partner), 3 doubles
     double *part;                // per (item, round) partial sums, 6 doubles
     const double *t0;            // per system: t_eph (hi+lo) at the start of integrate
+    // single-thread-order systems: the field of each body's geopotential and
+    // ring at every partner of its system, formed before the main force
+    // (rows of 3 doubles per partner; -1: the body has no such model), or null
+    const int32_t *srow_gp, *srow_ring;
+    const double *san;
 };
 
 void set_rk_table(const double *tab650, hipStream_t st);

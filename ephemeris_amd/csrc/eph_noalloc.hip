@@ -216,6 +216,9 @@ extern "C" int eph_accel_noalloc(const eph_mass *masses, int64_t n, const eph_ge
     v.np_ring = dp->np_ring;
     v.bsys = dp->bsys;
     v.perm = dp->perm;
+    v.srow_gp = nullptr;
+    v.srow_ring = nullptr;
+    v.san = nullptr;
     v.an = (double *)(base + kOffAn);
     v.part = (double *)(base + kOffPart);
     v.t0 = &dp->t0;

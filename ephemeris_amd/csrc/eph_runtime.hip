@@ -198,6 +198,9 @@ int Batch::build_plan(int mode) {
     std::vector<NPItem> items;
     std::vector<int2> blk_pool, blk_serial, blk_recv, work_pool;
     int64_t an_rows = 0;
+    std::vector<NPItem> sitems;
+    std::vector<int32_t> srow_gp, srow_ring;
+    int64_t san_rows = 0;
     pl.maxT = 1;
     pl.max_n = 1;
     pl.np_fusable = true;
@@ -216,6 +219,19 @@ int Batch::build_plan(int mode) {
         m.serial = (mode == 1 || n_task <= 1) ? 1 : 0;
         if (m.serial) {
             append_blocks(blk_serial, (int)s, (int)n);
+            // each model's field at every partner, formed in parallel before
+            // the main force (the single-thread kernel reads it per partner)
+            for (int64_t i = 0; i < n; ++i) {
+                const int64_t g = sys_off[s] + i;
+                for (int kind = 0; kind < 2; ++kind) {
+                    const int32_t model = kind == 0 ? host_gp[g] : host_ring[g];
+                    if (model < 0) continue;
+                    if (srow_gp.empty()) { srow_gp.assign((size_t)nb, -1); srow_ring.assign((size_t)nb, -1); }
+                    sitems.push_back(NPItem{(int32_t)g, kind, model, 0, (int32_t)s, (int32_t)san_rows});
+                    (kind == 0 ? srow_gp : srow_ring)[(size_t)g] = (int32_t)san_rows;
+                    san_rows += n;
+                }
+            }
             continue;
         }
         n_task = std::min<int64_t>(n_task, 64);
@@ -273,6 +289,14 @@ int Batch::build_plan(int mode) {
     pl.n_recv = (int)blk_recv.size();
     pl.nrows = (int)items.size();
     pl.an_rows = an_rows;
+    pl.n_sitems = (int)sitems.size();
+    if (!sitems.empty()) {
+        if ((rc = dev_upload(*this, pl.d_sitems, sitems))) return rc;
+        if ((rc = dev_upload(*this, pl.d_srow_gp, srow_gp))) return rc;
+        if ((rc = dev_upload(*this, pl.d_srow_ring, srow_ring))) return rc;
+        HIP_CHECK(hipMalloc(&pl.d_san, sizeof(double) * 3 * san_rows));
+        dev_allocs.push_back(pl.d_san);
+    }
     if (an_rows) {
         HIP_CHECK(hipMalloc(&pl.d_an, sizeof(double) * 3 * an_rows));
         dev_allocs.push_back(pl.d_an);
@@ -296,6 +320,9 @@ View Batch::view(int mode) const {
     v.np_ring = d_np_ring;
     v.bsys = d_bsys;
     v.perm = d_perm;
+    v.srow_gp = pl.d_srow_gp;
+    v.srow_ring = pl.d_srow_ring;
+    v.san = pl.d_san;
     v.an = pl.d_an;
     v.part = pl.d_part;
     v.t0 = d_t0;
@@ -352,6 +379,12 @@ int Batch::accel(int mode, bool prepare_only) {
     }
     if (pl.n_serial) {
         Scope sc(profiler, 3, stream);
+        if (pl.n_sitems) {                                 // the models' fields at every partner
+            View vs = v;
+            vs.items = pl.d_sitems;
+            vs.an = pl.d_san;
+            launch_np_field(vs, pl.n_sitems, pl.max_n, stream);
+        }
         launch_main_serial(v, pl.d_blk_serial, pl.n_serial, stream);
     }
     if (pl.nrows && !fused_np) {

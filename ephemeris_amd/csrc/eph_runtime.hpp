@@ -55,6 +55,11 @@ struct Plan {
     bool np_fusable = true;          // every pool system fits the fused non-point pass
     int2 *d_blk16 = nullptr;         // (system, first body) of 16-body blocks, largest systems first
     int n_blk16 = 0;
+    // item-field rows of the single-thread-order systems (see View::san)
+    NPItem *d_sitems = nullptr;
+    int n_sitems = 0;
+    int32_t *d_srow_gp = nullptr, *d_srow_ring = nullptr;
+    double *d_san = nullptr;
 };
 
 struct Batch {
