@@ -120,6 +120,25 @@ def test_integrate_full504_one_day():
     assert dr <= POS_TOL and dv <= VEL_TOL
 
 
+def test_integrate_full504_single_thread_order():
+    """RK12 in the single-thread summation order (mass.cpp:309-374) on the GPU,
+    whose main force reads the models' fields from rows formed beforehand,
+    against the oracle integrating in the same order."""
+    print("\n[case] integrate full504 3 steps, single-thread order", flush=True)
+    from oracle import pyoracle as po
+    from tests.util import CONFIGS
+    d = load_npz(GOLDEN / "accel_full504_serial.npz")
+    o = po.OSystem.load(CONFIGS["full504"], pool_at_load=False)
+    g = GpuSystems(o.masses.copy(), None, d["gps"], d["rings"])     # the same bodies on both sides
+    g.integrate(300.0, 3, _lib.SUMMATION_SERIAL)
+    m, _ = g.download()
+    o.integrate(300.0, 3, pool=False)
+    ref = o.masses.copy()
+    dr, dv, dw, exact = _traj_compare(m, ref)
+    print(f"full504 3 steps serial: max rel dr={dr:.3e} dv={dv:.3e} |dw|={dw:.3e} exact hi fields={exact}/6", flush=True)
+    assert dr <= 1e-15 and dv <= 1e-13
+
+
 def test_integrate_is_deterministic_and_chunk_consistent():
     print("\n[case] determinism small25", flush=True)
     d = load_npz(GOLDEN / "accel_small25_serial.npz")
