@@ -1442,25 +1442,27 @@ __global__ __launch_bounds__(kBS) void k_np_recv(View d, const int2 *__restrict_
     if (jl >= n) return;
     const int gj = sm.off + jl;
     v3 da = A.ldv(F_DACC, gj), dq = A.ldv(F_DTQ, gj);
-    int sj = 0;
-    while ((sj + 1) * n / T <= jl) ++sj;
-    for (int k = 0; k < T; ++k) {
-        int c = sj - k;
-        if (c < 0) c += T;
-        const int first = c * nit / T, last = (c + 1) * nit / T;
-        for (int q = last - 1; q >= first; --q) {
-            int body, an_off;
-            double gm;
-            if (staged) {
-                body = s_body[q]; an_off = s_an[q]; gm = s_gm[q];
-            } else {
-                const NPItem it = d.items[sm.item0 + q];
-                body = it.body; an_off = it.an_off; gm = A.ld(F_GM, it.body);
-            }
-            if (body == gj) continue;
-            const double *a = d.an + 3 * ((int64_t)an_off + jl);
-            da += gm * v3{a[0], a[1], a[2]};
+    // rounds k = 0..T-1 visit item chunks sj, sj-1, ... (mod T), each from
+    // its last item down (sj: the body's partner chunk): one cyclic
+    // descending pass over the items from the last item of chunk sj.  The
+    // item sequence is known in advance, so the fields are loaded ahead of
+    // the ordered additions.
+    const int sj = ((jl + 1) * T - 1) / n;
+    int q = (sj + 1) * nit / T - 1;
+#pragma unroll 4
+    for (int m = 0; m < nit; ++m) {
+        int body, an_off;
+        double gm;
+        if (staged) {
+            body = s_body[q]; an_off = s_an[q]; gm = s_gm[q];
+        } else {
+            const NPItem it = d.items[sm.item0 + q];
+            body = it.body; an_off = it.an_off; gm = A.ld(F_GM, it.body);
         }
+        const double *a = d.an + 3 * ((int64_t)an_off + jl);
+        const v3 an{a[0], a[1], a[2]};
+        if (body != gj) da += gm * an;
+        q = q == 0 ? nit - 1 : q - 1;
     }
     // own items: idaccel (geopotential) then jdaccel (ring), mass.cpp:298-307
     for (int kind = 0; kind < 2; ++kind) {
