@@ -54,6 +54,7 @@ SIGNATURES = {
     "eph_server_stop": (I, [P]),
     "eph_server_job_info": (I, [P, I64, P]),
     "eph_server_result_ensemble": (I, [P, I64, P, P]),
+    "eph_server_snapshot_ensemble": (I, [P, I64, P, P, C.POINTER(C.c_int64)]),
     "eph_server_result_particles": (I, [P, I64, P, P, P]),
     "eph_server_release": (I, [P, I64]),
     "eph_server_stats_read": (I, [P, P]),

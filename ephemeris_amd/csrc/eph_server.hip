@@ -723,6 +723,54 @@ int eph_server_result_ensemble(eph_server *srv, int64_t job, eph_mass *masses, d
     return EPH_OK;
 }
 
+int eph_server_snapshot_ensemble(eph_server *srv, int64_t job, eph_mass *masses, double *t_eph, int64_t *steps_done) {
+    if (!srv) return fail(EPH_EINVAL, "null server");
+    Server *s = S(srv);
+    // between ticks of the ensemble loop the resident batch holds whole quanta
+    std::lock_guard<std::mutex> tk(s->tick_mu[EPH_JOB_ENSEMBLE]);
+    SJob *j = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->jobs.find(job);
+        if (it == s->jobs.end()) return fail(EPH_EINVAL, "unknown job id");
+        j = it->second.get();
+        if (j->kind != EPH_JOB_ENSEMBLE) return fail(EPH_EINVAL, "job is of the other kind");
+        if (j->status == EPH_JOB_FAILED) return fail(EPH_EINVAL, "job failed: " + j->error);
+        if (j->status != EPH_JOB_RUNNING) {               // queued: its input; done: its result
+            if (masses) std::copy(j->masses.begin(), j->masses.end(), masses);
+            if (t_eph) std::copy(j->t_eph.begin(), j->t_eph.end(), t_eph);
+            if (steps_done) *steps_done = j->steps_done;
+            return EPH_OK;
+        }
+    }
+    // running: its systems in the resident batch
+    HIP_CHECK(hipSetDevice(s->device));
+    std::vector<eph_mass> all((size_t)s->ens->nb);
+    int rc = s->ens->download(all.data());
+    if (rc) return rc;
+    int64_t s0 = -1;
+    for (int64_t q = 0; q < s->ens->n_sys && s0 < 0; ++q)
+        if (s->ens_owner[(size_t)q] == j->id) s0 = q;
+    if (s0 < 0) return fail(EPH_EDEVICE, "running job not found in the resident batch");
+    int64_t k = 0;
+    for (int64_t q = s0; q < s0 + (int64_t)j->sys_n.size(); ++q) {
+        for (int64_t g = s->ens->sys_off[(size_t)q]; g < s->ens->sys_off[(size_t)q + 1]; ++g, ++k) {
+            if (!masses) continue;
+            eph_mass m = all[(size_t)g];
+            m.gp = j->masses[(size_t)k].gp;                // model indices local to the job
+            m.ring = j->masses[(size_t)k].ring;
+            masses[k] = m;
+        }
+        if (t_eph) {
+            t_eph[2 * (q - s0)] = s->ens->t_eph[(size_t)q].hi;
+            t_eph[2 * (q - s0) + 1] = s->ens->t_eph[(size_t)q].lo;
+        }
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (steps_done) *steps_done = j->steps_done;
+    return EPH_OK;
+}
+
 int eph_server_result_particles(eph_server *srv, int64_t job, double *r, double *v, double *extrema) {
     if (!srv) return fail(EPH_EINVAL, "null server");
     Server *s = S(srv);

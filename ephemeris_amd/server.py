@@ -131,6 +131,16 @@ class Server:
         _lib.check(_lib.lib().eph_server_result_ensemble(self.h, int(job), _ptr(m), _ptr(t)))
         return m, t
 
+    def snapshot_ensemble(self, job):
+        """The job's systems at its last whole quantum and the steps done so far
+        (its input while queued, its result once done)."""
+        kind, nb, ns = self._shape[job]
+        m = np.zeros(nb, MASS_DTYPE)
+        t = np.zeros((ns, 2))
+        steps = C.c_int64()
+        _lib.check(_lib.lib().eph_server_snapshot_ensemble(self.h, int(job), _ptr(m), _ptr(t), C.byref(steps)))
+        return m, t, steps.value
+
     def result_particles(self, job):
         kind, n, _ = self._shape[job]
         r = np.zeros((n, 6))

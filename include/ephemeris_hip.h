@@ -250,6 +250,15 @@ int eph_server_job_info(eph_server *srv, int64_t job, eph_job_info *info);
 /* results of a DONE job, layouts as eph_system_download / eph_particles_download */
 int eph_server_result_ensemble(eph_server *srv, int64_t job, eph_mass *masses, double *t_eph);
 int eph_server_result_particles(eph_server *srv, int64_t job, double *r, double *v, double *extrema);
+/* State of an ensemble job at the last whole quantum (queued: its input;
+ * running: read from the resident batch between ticks; done: its result),
+ * with the steps done so far.  A caller writes it as the reference's binary
+ * checkpoint (eph_msystem_create, eph_msystem_analyse with reconstruct = 1,
+ * eph_msystem_save_checkpoint; one file per member) and resumes from it by
+ * loading the file and submitting the rest of the span.  No counterpart in
+ * the reference, whose checkpoints are written by msystem::save_checkpoint
+ * (src/modules/mass_loader.cpp:715-759). */
+int eph_server_snapshot_ensemble(eph_server *srv, int64_t job, eph_mass *masses, double *t_eph, int64_t *steps_done);
 int eph_server_release(eph_server *srv, int64_t job);       /* forget a finished job */
 int eph_server_stats_read(eph_server *srv, eph_server_stats *st);
 int eph_server_destroy(eph_server *srv);

@@ -163,3 +163,48 @@ def test_mixed_kinds_wait_under_a_tight_budget(setup):
     ma, ta = _alone_ensemble(one, 6)
     assert bits_equal(m, ma) and bits_equal(t, ta)
     srv.close()
+
+
+def test_snapshot_checkpoint_resume(setup, tmp_path):
+    """A running job's snapshot is its state after the quanta done so far; a
+    member written from it as a checkpoint, loaded again and integrated for
+    the rest of the span ends where the served job ends."""
+    cfg, members, r, v = setup
+    print("\n[case] server snapshot: queued, running after one quantum, done; checkpoint resume", flush=True)
+    srv = Server(_field(cfg), ens_dt=ENS_DT, ens_quantum=ENS_Q, part_dt=PART_DT, part_quantum=PART_Q,
+                 part_max_steps=PART_MAX)
+    other = _job(members, [0])
+    srv.submit_ensemble(*other, n_steps=2 * ENS_Q)          # ahead of the job in the resident batch
+    job = _job(members, [1, 2])
+    jid = srv.submit_ensemble(*job, n_steps=3 * ENS_Q)
+    m0, t0, k0 = srv.snapshot_ensemble(jid)
+    assert k0 == 0 and bits_equal(m0, job[0]) and bits_equal(t0, job[4])
+    srv.tick()
+    m1, t1, k1 = srv.snapshot_ensemble(jid)
+    ma, ta = _alone_ensemble(job, ENS_Q)
+    assert k1 == ENS_Q
+    assert not [f for f in m1.dtype.names if not bits_equal(m1[f], ma[f])]
+    assert bits_equal(t1, ta)
+    # the first member of the job as a checkpoint, resumed for the remaining steps
+    n0 = int(job[1][0])
+    ckpt = tmp_path / "member0.ckpt"
+    w = MSystem.from_masses(m1[:n0], job[2], job[3], tuple(t1[0]))
+    w.analyse(True)                                         # a checkpoint holds the orbit structure
+    w.save_checkpoint(ckpt)
+    w.close()
+    back = MSystem.from_checkpoint(ckpt)
+    gps, rings = back.models()
+    resumed = GpuSystems(back.masses, None, gps, rings, [back.t_eph()])
+    back.close()
+    for _ in range(2):
+        resumed.integrate(ENS_DT, ENS_Q)
+    mr, tr = resumed.download()
+    srv.run_until_idle()
+    m3, t3, k3 = srv.snapshot_ensemble(jid)
+    mf, tf = srv.result_ensemble(jid)
+    assert k3 == 3 * ENS_Q and bits_equal(m3, mf) and bits_equal(t3, tf)
+    keep = [f for f in mf.dtype.names if f not in ("gp", "ring")]
+    bad_f = [f for f in keep if not bits_equal(mr[f], mf[:n0][f])]
+    print(f"resumed member: {n0} bodies, mismatching fields {bad_f}", flush=True)
+    assert not bad_f and bits_equal(tr[0], tf[0])
+    srv.close()
