@@ -40,6 +40,8 @@ SIGNATURES = {
     "eph_system_set_graph": (I, [P, I]),
     "eph_system_set_autotune": (I, [P, I]),
     "eph_system_kernel_variant": (I, [P]),
+    "eph_system_set_waves": (I, [P, I]),
+    "eph_system_kernel_waves": (I, [P]),
     "eph_particles_create": (I, [P, I64, P, P, D, D, PP]),
     "eph_particles_integrate": (I, [P, D, I64]),
     "eph_particles_download": (I, [P, P, P, P]),

@@ -1997,13 +1997,15 @@ void launch_main_k4(const View &d, const int2 *blk16, int nblk, hipStream_t st) 
 // (52 KB of LDS each: three per CU): the pair terms of a 504-body system are
 // formed in two rounds instead of four (38 us instead of 44 us per launch on
 // one MI355X); the ordered sums are the same.
-void launch_main_wpb(const View &d, const int2 *work, int nwork, int max_n, bool fused_np, hipStream_t st) {
+void launch_main_wpb(const View &d, const int2 *work, int nwork, int max_n, bool fused_np, int waves, hipStream_t st) {
     if (!nwork) return;
-    const bool four = max_n > 128 && nwork <= 768;
-    if (fused_np && four) hipLaunchKernelGGL((k_main_wpb<true, 4>), dim3(nwork), dim3(256), 0, st, d, work);
+    int W = waves;
+    if (W == 0) W = (max_n > 128 && nwork <= 768) ? 4 : (max_n > 64 && nwork <= 1536) ? 2 : 1;
+    if (fused_np && W == 1) W = 2;
+    if (fused_np && W == 4) hipLaunchKernelGGL((k_main_wpb<true, 4>), dim3(nwork), dim3(256), 0, st, d, work);
     else if (fused_np) hipLaunchKernelGGL((k_main_wpb<true, 2>), dim3(nwork), dim3(128), 0, st, d, work);
-    else if (four) hipLaunchKernelGGL((k_main_wpb<false, 4>), dim3(nwork), dim3(256), 0, st, d, work);
-    else if (max_n > 64 && nwork <= 1536) hipLaunchKernelGGL((k_main_wpb<false, 2>), dim3(nwork), dim3(128), 0, st, d, work);
+    else if (W == 4) hipLaunchKernelGGL((k_main_wpb<false, 4>), dim3(nwork), dim3(256), 0, st, d, work);
+    else if (W == 2) hipLaunchKernelGGL((k_main_wpb<false, 2>), dim3(nwork), dim3(128), 0, st, d, work);
     else hipLaunchKernelGGL((k_main_wpb<false, 1>), dim3(nwork), dim3(64), 0, st, d, work);
 }
 // Wavefronts per body: several while every body's workgroup still fits on
@@ -2012,13 +2014,12 @@ void launch_main_wpb(const View &d, const int2 *work, int nwork, int max_n, bool
 // wavefront per body).  Four, not eight, for a 504-body system: two 256-lane
 // workgroups then share each CU, one round of workgroups instead of two
 // (31 us instead of 43-58 us per launch on one MI355X).
-void launch_deform_wpb(const View &d, const int2 *work, int nwork, int max_n, int prepare, int np_rows,
+void launch_deform_wpb(const View &d, const int2 *work, int nwork, int max_n, int prepare, int np_rows, int waves,
                        hipStream_t st) {
     if (!nwork && !np_rows) return;
     const int chunks = (max_n + 63) / 64;
-    int W = 1;
-    if (chunks >= 4 && nwork <= 1280) W = 4;
-    else if (chunks >= 2 && nwork <= 2816) W = 2;
+    int W = waves;
+    if (W == 0) W = (chunks >= 4 && nwork <= 1280) ? 4 : (chunks >= 2 && nwork <= 2816) ? 2 : 1;
     const NpTail np{nwork, np_rows, (max_n + 63) / 64};
     const dim3 grid(nwork + (np_rows * np.chunks + W - 1) / W), block(64 * W);
     if (np_rows) {

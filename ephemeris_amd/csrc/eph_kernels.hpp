@@ -66,14 +66,17 @@ void launch_main_serial(const View &d, const int2 *blk, int nblk, hipStream_t st
 // each body's workgroup, after the item fields are formed (launch_np_field or
 // the extra workgroups of a deform launch; every thread-pool-order system
 // has at most 512 bodies and 64 non-point items)
-void launch_main_wpb(const View &d, const int2 *work, int nwork, int max_n, bool fused_np, hipStream_t st);
+// waves: wavefronts per body, 0 = chosen from the batch's shape, else 1, 2 or
+// 4 (the fused kernel has at least 2)
+void launch_main_wpb(const View &d, const int2 *work, int nwork, int max_n, bool fused_np, int waves, hipStream_t st);
 void launch_np_field(const View &d, int nrows, int max_n, hipStream_t st);
 constexpr int kNpFusedMaxN_h = 512, kNpFusedMaxItems_h = 64;
 // one workgroup of up to 4 wavefronts per body (partner chunks evaluated in
 // parallel, summed in order); max_n = largest system of the batch
 // np_rows > 0: the launch also evaluates the non-point item fields
 // (launch_np_field) in extra workgroups
-void launch_deform_wpb(const View &d, const int2 *work, int nwork, int max_n, int prepare, int np_rows,
+// (waves as for launch_main_wpb)
+void launch_deform_wpb(const View &d, const int2 *work, int nwork, int max_n, int prepare, int np_rows, int waves,
                        hipStream_t st);
 void launch_np(const View &d, int nrows, int max_n, int maxT, const int2 *recv_blk, int nrecv, bool with_field,
                hipStream_t st);

@@ -352,7 +352,7 @@ int Batch::accel(int mode, bool prepare_only) {
     {
         Scope sc(profiler, 1, stream);
         if (wpb) {
-            launch_deform_wpb(v, d_work_all, n_work_all, max_sys_n, prepare_only ? 0 : 1, np_rows, stream);
+            launch_deform_wpb(v, d_work_all, n_work_all, max_sys_n, prepare_only ? 0 : 1, np_rows, eff_waves(), stream);
             field_done = true;
         } else if (variant == 1) {
             launch_deform(v, d_blk_light, n_blk_light, d_blk_damped, n_blk_damped, prepare_only ? 0 : 1, stream);
@@ -360,7 +360,7 @@ int Batch::accel(int mode, bool prepare_only) {
             // constant-time-lag bodies re-sum their partners once per spin pass:
             // one wavefront each; the rest four lanes each (that launch
             // carries the item fields)
-            launch_deform_wpb(v, d_work_damped, n_work_damped, max_sys_n, prepare_only ? 0 : 1, 0, stream);
+            launch_deform_wpb(v, d_work_damped, n_work_damped, max_sys_n, prepare_only ? 0 : 1, 0, 0, stream);
             launch_deform_k4(v, d_blk_light16, n_blk_light16, prepare_only ? 0 : 1, np_rows, pl.max_n, stream);
             field_done = true;
         }
@@ -373,7 +373,7 @@ int Batch::accel(int mode, bool prepare_only) {
     }
     if (pl.n_pool) {
         Scope sc(profiler, 2, stream);
-        if (wpb) launch_main_wpb(v, pl.d_work_pool, pl.n_work_pool, pl.max_n, fused_np, stream);
+        if (wpb) launch_main_wpb(v, pl.d_work_pool, pl.n_work_pool, pl.max_n, fused_np, eff_waves(), stream);
         else if (variant == 1) launch_main_pool(v, pl.d_blk_pool, pl.n_pool, stream);
         else launch_main_k4(v, pl.d_blk16, pl.n_blk16, stream);
     }
@@ -451,7 +451,7 @@ void Batch::drop_graph() {
 // counter) captured as a graph, rebuilt when dt, the summation order or the
 // kernel choice changes.
 int Batch::step_graph(double dt, int mode) {
-    if (gexec && g_dt == dt && g_mode == mode && g_variant == eff_variant()) return EPH_OK;
+    if (gexec && g_dt == dt && g_mode == mode && g_variant == eff_variant() && g_waves == eff_waves()) return EPH_OK;
     drop_graph();
     if (!d_istep) {
         HIP_CHECK(hipMalloc(&d_istep, sizeof(int64_t)));
@@ -479,19 +479,24 @@ int Batch::step_graph(double dt, int mode) {
     g_dt = dt;
     g_mode = mode;
     g_variant = eff_variant();
+    g_waves = eff_waves();
     return EPH_OK;
 }
 
 // ---- autotuner ------------------------------------------------------------------
 // For a batch in automatic mode with enough thread-pool-order bodies, time
-// one thread per body, one wavefront per body and four lanes per body on a
-// device copy of the batch (results are bit-identical, so only speed
-// differs) and keep the fastest; choices are cached per shape bucket for the
+// the block shapes of the force passes on a device copy of the batch --
+// one thread per body (64-body blocks, 64-partner j-tiles), one workgroup
+// of W = 1, 2 or 4 wavefronts per body (j-tiles of 64 W partners per
+// round), four lanes per body (16 bodies per wavefront, 4-partner groups)
+// -- and keep the fastest (results are bit-identical, so only speed
+// differs).  W = 2 and 4 are tried only while their workgroups can still
+// share the GPU in few rounds; choices are cached per shape bucket for the
 // life of the process.
 namespace {
 constexpr int kTuneMinBodies = 2048;
 std::mutex g_tune_mu;
-std::map<std::array<int, 4>, int> g_tune_cache;
+std::map<std::array<int, 4>, std::pair<int, int>> g_tune_cache;
 int log2_bucket(int64_t x) { int b = 0; while (x > 1) { x >>= 1; ++b; } return b; }
 }  // namespace
 
@@ -504,20 +509,29 @@ int Batch::autotune(double dt, int mode) {
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune_cache.find(key);
-        if (it != g_tune_cache.end()) { tuned = it->second; return EPH_OK; }
+        if (it != g_tune_cache.end()) {
+            tuned = it->second.first;
+            tuned_waves = it->second.second;
+            return EPH_OK;
+        }
     }
     std::vector<int> all((size_t)n_sys);
     for (int64_t k = 0; k < n_sys; ++k) all[k] = (int)k;
-    int best = 0;
+    std::vector<std::pair<int, int>> cands{{1, 0}, {2, 1}};   // (variant, wavefronts per body)
+    if (pl.n_work_pool <= 8192) cands.push_back({2, 2});
+    if (pl.n_work_pool <= 4096) cands.push_back({2, 4});
+    cands.push_back({4, 0});
+    std::pair<int, int> best{0, 0};
     double best_s = 1e300;
-    for (int cand : {1, 2, 4}) {
+    for (const auto &cand : cands) {
         Batch *c = nullptr;
         if (regroup_batch(this, all, 0, nullptr, nullptr, nullptr, 0, nullptr, 0, nullptr, &c) || !c) {
             g_last_error.clear();                          // not enough memory for a copy: keep the heuristic
             return EPH_OK;
         }
         std::unique_ptr<Batch> own(c);
-        c->variant = cand;
+        c->variant = cand.first;
+        c->waves = cand.second;
         c->use_graph = use_graph;
         c->force_generic = true;
         c->tune_done = true;
@@ -533,7 +547,8 @@ int Batch::autotune(double dt, int mode) {
         if (rc) return rc;
         if (el < best_s) { best_s = el; best = cand; }
     }
-    tuned = best;
+    tuned = best.first;
+    tuned_waves = best.second;
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune_cache[key] = best;
     return EPH_OK;
@@ -1138,6 +1153,20 @@ int eph_system_set_autotune(eph_system *sys, int enable) {
 int eph_system_kernel_variant(const eph_system *sys) {
     if (!sys) return fail(EPH_EINVAL, "null system");
     return B(sys)->eff_variant();
+}
+
+int eph_system_set_waves(eph_system *sys, int waves_per_body) {
+    if (!sys) return fail(EPH_EINVAL, "null system");
+    if (waves_per_body != 0 && waves_per_body != 1 && waves_per_body != 2 && waves_per_body != 4)
+        return fail(EPH_EINVAL, "wavefronts per body must be 0 (automatic), 1, 2 or 4");
+    B(sys)->waves = waves_per_body;
+    B(sys)->drop_graph();
+    return EPH_OK;
+}
+
+int eph_system_kernel_waves(const eph_system *sys) {
+    if (!sys) return fail(EPH_EINVAL, "null system");
+    return B(sys)->eff_waves();
 }
 
 int eph_system_set_graph(eph_system *sys, int enable) {

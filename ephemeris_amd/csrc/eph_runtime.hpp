@@ -97,12 +97,17 @@ struct Batch {
     Profiler profiler;
     bool force_generic = false;  // never take the persistent small path
     // kernel variant of the j-loop passes: 0 = automatic, 1 = one thread per
-    // body, 2 = one wavefront per body
+    // body, 2 = one workgroup per body (3: the same without the fused
+    // non-point sums), 4 = four lanes per body
     int variant = 0;
-    // automatic choice made by the autotuner (0: static heuristic)
-    int tuned = 0;
+    // wavefronts per body of the one-workgroup-per-body kernels: 0 =
+    // automatic, else 1, 2 or 4
+    int waves = 0;
+    // automatic choices made by the autotuner (0: static heuristic)
+    int tuned = 0, tuned_waves = 0;
     bool use_autotune = true, tune_done = false;
     int eff_variant() const { return variant ? variant : tuned; }
+    int eff_waves() const { return waves ? waves : (variant ? 0 : tuned_waves); }
     int autotune(double dt, int mode);
     int2 *d_work_all = nullptr;
     int n_work_all = 0;
@@ -126,7 +131,7 @@ struct Batch {
     bool use_graph = true;
     hipGraphExec_t gexec = nullptr;
     double g_dt = 0;
-    int g_mode = -1, g_variant = -1;
+    int g_mode = -1, g_variant = -1, g_waves = -1;
     int64_t *d_istep = nullptr;
     int step_graph(double dt, int mode);
     void drop_graph();

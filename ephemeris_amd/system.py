@@ -86,6 +86,13 @@ class GpuSystems:
     def kernel_variant(self):
         return _lib.check(_lib.lib().eph_system_kernel_variant(self.h))
 
+    def set_waves(self, waves_per_body=0):
+        """Wavefronts per body of variants 2 and 3: 0 automatic, 1, 2 or 4."""
+        _lib.check(_lib.lib().eph_system_set_waves(self.h, int(waves_per_body)))
+
+    def kernel_waves(self):
+        return _lib.check(_lib.lib().eph_system_kernel_waves(self.h))
+
     def set_graph(self, enable=True):
         """Replay each RK12 step as one captured HIP graph (default on)."""
         _lib.check(_lib.lib().eph_system_set_graph(self.h, int(bool(enable))))

@@ -153,11 +153,21 @@ int eph_accel_noalloc(const eph_mass *masses, int64_t n, const eph_geopotential 
  * No counterpart in the reference (its only knob is CUDA_CORES,
  * src/integrators/RungeKutta.cu:13-18, which changes results). */
 int eph_system_set_variant(eph_system *sys, int variant, int allow_persistent);
+/* Wavefronts per body of variants 2 and 3 (the j-tile of a round is 64 x
+ * that many partners): 0 automatic (from the batch's shape), 1, 2 or 4.
+ * The kernel with the fused non-point pass uses at least 2.  Results do not
+ * depend on it.  eph_system_kernel_waves returns the value in use (0: chosen
+ * per launch from the batch's shape).  No counterpart in the reference. */
+int eph_system_set_waves(eph_system *sys, int waves_per_body);
+int eph_system_kernel_waves(const eph_system *sys);
 /* Autotuner (default on): in automatic mode, the first eph_integrate of a
- * batch with >= 2048 thread-pool-order bodies times variants 1, 2 and 4 for
- * a few steps on a device copy of the batch and keeps the fastest (the choice
- * is cached per batch shape for the process).  Results do not depend on it.
- * eph_system_kernel_variant returns the variant in use (0: heuristic). */
+ * batch with >= 2048 thread-pool-order bodies times the block shapes -- one
+ * thread per body (variant 1), one workgroup of 1, 2 or 4 wavefronts per
+ * body (variant 2; 2 and 4 only up to 8192 and 4096 bodies), four lanes per
+ * body (variant 4) -- for a few steps on a device copy of the batch and
+ * keeps the fastest (the choice is cached per batch shape for the process).
+ * Results do not depend on it.  eph_system_kernel_variant returns the
+ * variant in use (0: heuristic). */
 int eph_system_set_autotune(eph_system *sys, int enable);
 int eph_system_kernel_variant(const eph_system *sys);
 /* enable = 1 (default): eph_integrate runs each RK12 step as one replay of a

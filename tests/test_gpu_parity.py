@@ -169,6 +169,38 @@ def test_kernel_variants_bitwise(name, tag):
         assert not diff, (variant, diff)
 
 
+@pytest.mark.parametrize("name", ["full504", "solar504"])
+def test_wavefronts_per_body_bitwise(name):
+    """One, two or four wavefronts per body in the one-workgroup-per-body
+    kernels (j-tiles of 64, 128 or 256 partners per round), with and without
+    the fused non-point pass: the bits of one thread per body, for a force
+    evaluation and for four RK12 steps."""
+    print(f"\n[case] wavefronts per body 1/2/4, variants 2 and 3, {name} pool", flush=True)
+    d = load_npz(GOLDEN / f"accel_{name}_pool.npz")
+    ref = _gpu(d)
+    ref.set_variant(1, allow_persistent=False)
+    ref.accel(_lib.SUMMATION_POOL)
+    ra, _ = ref.download()
+    ref.integrate(300.0, 4)
+    rb, tb = ref.download()
+    for variant in (2, 3):
+        for waves in (1, 2, 4):
+            g = _gpu(d)
+            g.set_variant(variant, allow_persistent=False)
+            g.set_waves(waves)
+            assert g.kernel_waves() == waves
+            g.accel(_lib.SUMMATION_POOL)
+            a, _ = g.download()
+            bad = [f for f in a.dtype.names if not bits_equal(a[f], ra[f])]
+            assert not bad, (variant, waves, "accel", bad)
+            g.integrate(300.0, 4)
+            b, t = g.download()
+            bad = [f for f in b.dtype.names if not bits_equal(b[f], rb[f])]
+            assert not bad and bits_equal(t, tb), (variant, waves, "integrate", bad)
+    with pytest.raises(_lib.EphError):
+        ref.set_waves(3)
+
+
 @pytest.mark.parametrize("variant,persistent", [(1, False), (2, False), (3, False), (4, False), (0, True)])
 def test_integrate_variants_agree(variant, persistent):
     print(f"\n[case] integrate full504 and small25 12 steps, variant {variant}", flush=True)
