@@ -336,6 +336,20 @@ def _zonal_terms(rng, j2, nmax=6):
     return [(n, 0, j2 * rng.uniform(-0.05, 0.05) * 10.0 ** (-(n - 2)), 0.0) for n in range(3, nmax + 1)]
 
 
+def _asteroid(rng, star: Body, q: int) -> Body:
+    """Main-belt asteroid number q + 1: a uniform in 2.1-3.3 AU, e and i
+    Rayleigh (sigma 0.1), mass log-uniform in 1e15-1e21 kg, spinning."""
+    mass = _loguni(rng, 1e15, 1e21)
+    a = rng.uniform(2.1, 3.3) * AU
+    e = min(rng.rayleigh(0.1), 0.6)
+    inc = min(rng.rayleigh(0.1), 1.2)
+    Om, om, M = rng.uniform(0, 2 * math.pi, 3)
+    r, v, nrm = kepler_state(star.GM + G * mass, a, e, inc, Om, om, M)
+    x, z, w = _spin_frame(rng, rng.normal(size=3), rng.uniform(2.2, 30.0), 0.0)
+    return Body(f"Asteroid {q + 1}", f"A{q + 1:04d}", G * mass, _radius_km(mass, 2200.0),
+                r=r, v=v, x_axis=x, z_axis=z, w=w)
+
+
 def make_full(seed: int = 20251) -> SystemSpec:
     """504-body system with the full force model."""
     rng = np.random.default_rng(seed)
@@ -382,15 +396,7 @@ def make_full(seed: int = 20251) -> SystemSpec:
         bodies += _moons(rng, d, dm_alloc[k], f"D{k + 1}_", rmin=5.0, rmax=60.0,
                          ratio_lo=1e-5, ratio_hi=1e-1, spaced=True)
     for q in range(342):
-        mass = _loguni(rng, 1e15, 1e21)
-        a = rng.uniform(2.1, 3.3) * AU
-        e = min(rng.rayleigh(0.1), 0.6)
-        inc = min(rng.rayleigh(0.1), 1.2)
-        Om, om, M = rng.uniform(0, 2 * math.pi, 3)
-        r, v, nrm = kepler_state(star.GM + G * mass, a, e, inc, Om, om, M)
-        x, z, w = _spin_frame(rng, rng.normal(size=3), rng.uniform(2.2, 30.0), 0.0)
-        ast = Body(f"Asteroid {q + 1}", f"A{q + 1:04d}", G * mass, _radius_km(mass, 2200.0),
-                   r=r, v=v, x_axis=x, z_axis=z, w=w)
+        ast = _asteroid(rng, star, q)
         if q < 3:
             ast.extra = {"Receptance": 1.0}
         bodies.append(ast)
@@ -412,6 +418,23 @@ def make_full(seed: int = 20251) -> SystemSpec:
     assert len(bodies) == 504, len(bodies)
     _barycentre(bodies)
     return SystemSpec(bodies, delta_t=300.0)
+
+
+def add_asteroids(spec: SystemSpec, count: int, seed: int = 20254) -> SystemSpec:
+    """The system with `count` more main-belt asteroids (numbered after the
+    existing ones; positions relative to the first body, which is the star),
+    re-centred on the barycentre: systems larger than the 504-body one."""
+    rng = np.random.default_rng(seed)
+    bodies = [replace(b) for b in spec.bodies]
+    star = bodies[0]
+    n_ast = sum(b.sid.startswith("A") for b in bodies)
+    for q in range(n_ast, n_ast + count):
+        b = _asteroid(rng, star, q)
+        b.r = b.r + star.r
+        b.v = b.v + star.v
+        bodies.append(b)
+    _barycentre(bodies)
+    return SystemSpec(bodies, spec.delta_t, spec.tdb)
 
 
 def truncate(spec: SystemSpec, n: int) -> SystemSpec:
