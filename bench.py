@@ -58,8 +58,8 @@ def particle_flop_per_step(nf, n_quad, n_spin):
     from k_particles / particle_accel: per stage, the field bodies' potential
     and Newtonian pull (54 each), the reactions of the quadrupoles (88 each),
     RELATIVITY (108 each) and the partners' Lense-Thirring terms (31 each),
-    plus 25 fixed; and the stage combination, which re-forms each stage
-    velocity it uses from the stored stage derivatives."""
+    plus 25 fixed; and the stage combination, which forms each stage
+    velocity it uses from the stage sum kept when that stage was formed."""
     per_stage = nf * 54 + 16 + n_quad * 88 + nf * 108 + n_spin * 31 + 9
     rows = rk12_nonzero_rows()
     comb = 0
@@ -68,7 +68,7 @@ def particle_flop_per_step(nf, n_quad, n_spin):
             comb += 1 + 6                                          # fc, ca += fc * a_j
             comb += 3 * DD_MUL + 3 * DD_ADD                        # cv += c_kj * v_j (dd)
             if j > 0:
-                comb += 7 * len(rows[j]) + 3 + 3 * DD_ADD          # v_j = v0 + dt * sum a_jl a_l
+                comb += 3 + 3 * DD_ADD                             # v_j = v0 + dt * (kept sum_l a_jl a_l)
         comb += (3 * DD_MUL + 3 * DD_ADD) + (3 + 3 * DD_ADD)      # r_k, v_k
     return 25 * per_stage + comb
 

@@ -1831,13 +1831,15 @@ __device__ __forceinline__ v3 particle_accel(const JMain *__restrict__ T, int nf
 
 // Particle fields (stride pv.npp): r (6), v (6), last acceleration (3),
 // min_distance, max_influence; then, per lane of the launch, the derivatives
-// of stages kPartLdsStages..24 of the step in progress (3 rows each).
+// of stages kPartLdsStages..24 of the step in progress (3 rows each) and the
+// stage sums of stages 1..24 (3 rows each).
 enum PField : int { P_R = 0, P_V = 6, P_A = 12, P_MIND = 15, P_MAXI = 16, P_COUNT = 17, P_STAGE = 17 };
 // Stage derivatives 0..12 of a step live in LDS (312 B per particle: two
-// 256-particle workgroups per CU), 13..24 in the particle's scratch rows
-// (L2-resident); every access names its address space, so no access goes
-// through a generic pointer.
+// 256-particle workgroups per CU), 13..24 in the particle's scratch rows;
+// every access names its address space, so no access goes through a
+// generic pointer.
 constexpr int kPartLdsStages = 13;
+constexpr int kPartStageRows = 3 * (25 - kPartLdsStages);     // rows of derivatives 13..24
 typedef __attribute__((address_space(3))) double lds_f64;
 typedef __attribute__((address_space(1))) double glb_f64;
 
@@ -1889,24 +1891,16 @@ __global__ __launch_bounds__(kPartBS) void k_particles(PView pv, int n_steps, do
             fg[(r + 0) * S] = a.x; fg[(r + 1) * S] = a.y; fg[(r + 2) * S] = a.z;
         }
     };
-    // sum over the non-zero columns l of row j of (a_jl) * a_l, ascending l:
-    // the LDS columns first, then the scratch-row columns
-    auto row_sum = [&](int j) {
-        v3 caj{0, 0, 0};
-        const int bj = j * (j - 1) / 2;
-        const int f1 = c_row0[j + 1];
-        int f = c_row0[j];
-        for (; f < f1 && c_nzj[f] < kPartLdsStages; ++f) {
-            const int l = c_nzj[f];
-            const double2 cl = c_rk[bj + l];
-            caj += (cl.x + cl.y) * get_l(l);
-        }
-        for (; f < f1; ++f) {
-            const int l = c_nzj[f];
-            const double2 cl = c_rk[bj + l];
-            caj += (cl.x + cl.y) * get_g(l);
-        }
-        return caj;
+    // stage sums sum_l (a_jl) * a_l of stages j = 1..24, kept from stage j
+    // (the velocity of stage j is v0 + dt * that sum)
+    glb_f64 *fs = fg + (int64_t)kPartStageRows * S;
+    auto get_s = [&](int j) {
+        const int64_t r = (int64_t)(j - 1) * 3;
+        return v3{fs[(r + 0) * S], fs[(r + 1) * S], fs[(r + 2) * S]};
+    };
+    auto put_s = [&](int j, v3 c) {
+        const int64_t r = (int64_t)(j - 1) * 3;
+        fs[(r + 0) * S] = c.x; fs[(r + 1) * S] = c.y; fs[(r + 2) * S] = c.z;
     };
     v3 a{ld(P_A + 0), ld(P_A + 1), ld(P_A + 2)};
     for (int step = 0; step < n_steps; ++step) {
@@ -1914,8 +1908,8 @@ __global__ __launch_bounds__(kPartBS) void k_particles(PView pv, int n_steps, do
         put(0, a);
         for (int k = 1; k <= 25; ++k) {
             // x_k = x0 + dt * sum_j a_kj f_j (RungeKutta.impl:148-170); the
-            // velocity of stage j is v0 + dt * (sum_l a_jl a_l), recomputed
-            // from the stored accelerations in the same order
+            // velocity of stage j is v0 + dt * (sum_l a_jl a_l), that sum as
+            // stage j formed it (the same operations, so the same bits)
             ddv3 cv = mkddv(v3{0, 0, 0});
             v3 ca{0, 0, 0};
             const int base = k * (k - 1) / 2;
@@ -1927,7 +1921,7 @@ __global__ __launch_bounds__(kPartBS) void k_particles(PView pv, int n_steps, do
                 const dd ckj{c.x, c.y};
                 const double fc = c.x + c.y;
                 ddv3 vj = v0;
-                if (j > 0) vj = add(v0, mkddv(dt * row_sum(j)));
+                if (j > 0) vj = add(v0, mkddv(dt * get_s(j)));
                 cv = add(cv, scale(ckj, vj));
                 ca += fc * get_l(j);
             }
@@ -1936,10 +1930,11 @@ __global__ __launch_bounds__(kPartBS) void k_particles(PView pv, int n_steps, do
                 const double2 c = c_rk[base + j];
                 const dd ckj{c.x, c.y};
                 const double fc = c.x + c.y;
-                const ddv3 vj = add(v0, mkddv(dt * row_sum(j)));
+                const ddv3 vj = add(v0, mkddv(dt * get_s(j)));
                 cv = add(cv, scale(ckj, vj));
                 ca += fc * get_g(j);
             }
+            if (k < 25) put_s(k, ca);
             r = add(r0, scale(mkdd(dt), cv));
             v = add(v0, mkddv(dt * ca));
             const JMain *T = pv.rec + ((entry0 + 25 * (int64_t)step + (k - 1)) * pv.nrec_sys) * kSmallMax;
