@@ -1834,18 +1834,18 @@ __device__ __forceinline__ v3 particle_accel(const JMain *__restrict__ T, int nf
 // of stages kPartLdsStages..24 of the step in progress (3 rows each) and the
 // stage sums of stages 1..24 (3 rows each).
 enum PField : int { P_R = 0, P_V = 6, P_A = 12, P_MIND = 15, P_MAXI = 16, P_COUNT = 17, P_STAGE = 17 };
-// Stage derivatives 0..12 of a step live in LDS (312 B per particle: two
-// 256-particle workgroups per CU), 13..24 in the particle's scratch rows;
+// Stage derivatives 0..7 of a step live in LDS (192 B per particle: three
+// 256-particle workgroups per CU), 8..24 in the particle's scratch rows;
 // every access names its address space, so no access goes through a
 // generic pointer.
-constexpr int kPartLdsStages = 13;
-constexpr int kPartStageRows = 3 * (25 - kPartLdsStages);     // rows of derivatives 13..24
+static_assert(kPartLdsStages == 8, "the LDS share and the occupancy hint of k_particles go together");
 typedef __attribute__((address_space(3))) double lds_f64;
 typedef __attribute__((address_space(1))) double glb_f64;
 
 // n_steps > 0: advance; n_steps < 0: compute the first acceleration from
 // the records; 0: leave the workgroup's particles alone.
-__global__ __launch_bounds__(kPartBS) void k_particles(PView pv, int n_steps, double dt, DtK dtk,
+// Three wavefronts per SIMD (168 registers; a few values spill to scratch).
+__global__ __launch_bounds__(kPartBS) __attribute__((amdgpu_waves_per_eu(3))) void k_particles(PView pv, int n_steps, double dt, DtK dtk,
                                                        const PBlock *__restrict__ blocks) {
     const int p = blockIdx.x * kPartBS + (int)threadIdx.x;
     int64_t entry0 = 0;

@@ -15,7 +15,14 @@ constexpr int kSmallMax = 64;    // largest system integrated by one workgroup
 // sum of the non-zero tableau weights of each stage (dt_k), index 1..25
 struct DtK { double v[26]; };
 
-constexpr int kPartBS = 256;     // particles per workgroup (stage derivatives in LDS: 312 B each)
+constexpr int kPartBS = 256;     // particles per workgroup
+// A particle's stage derivatives 0..kPartLdsStages-1 of the step in progress
+// live in LDS (24 B each per particle), the rest and the stage sums of
+// stages 1..24 in per-particle scratch rows after the particle's state:
+// kPartScratchRows rows, which every particle pool allocates.
+constexpr int kPartLdsStages = 8;
+constexpr int kPartStageRows = 3 * (25 - kPartLdsStages);        // derivatives kPartLdsStages..24
+constexpr int kPartScratchRows = kPartStageRows + 3 * 24;          // and the stage sums 1..24
 
 // test particles in the recorded field of one small system
 struct PView {

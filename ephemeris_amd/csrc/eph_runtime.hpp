@@ -21,7 +21,7 @@ int fail(int code, const std::string &msg);
 
 constexpr int kNumKernelClasses = 7;
 constexpr int P_COUNT_HOST = 17;     // particle fields (PField)
-constexpr int P_STAGE_ROWS = 36 + 72;  // per-step scratch rows after them: stage derivatives 13..24, stage sums 1..24
+constexpr int P_STAGE_ROWS = kPartScratchRows;   // per-step scratch rows after them (k_particles)
 constexpr int P_ALLOC_ROWS = P_COUNT_HOST + P_STAGE_ROWS;
 extern const char *const kKernelNames[kNumKernelClasses];
 
