@@ -76,6 +76,7 @@ struct Server {
 
     std::mutex mu;                       // jobs, queue, counters
     std::mutex tick_mu[2];               // one tick of each kind at a time
+    std::atomic<int> snap_waiting{0};    // snapshot callers waiting for the ensemble tick lock
     std::map<int64_t, std::unique_ptr<SJob>> jobs;
     std::deque<int64_t> queue;
     int64_t next_id = 1;
@@ -643,6 +644,8 @@ int eph_server_start(eph_server *srv) {
                     if (s->stop_flag) break;
                     if (!s->has_work(kind)) continue;
                 }
+                // a snapshot caller waiting for the tick lock goes first
+                while (kind == EPH_JOB_ENSEMBLE && s->snap_waiting.load() > 0 && !s->stop_flag) std::this_thread::yield();
                 if (s->tick_kind(kind)) {
                     std::lock_guard<std::mutex> lk(s->mu);
                     s->worker_error = eph_last_error();
@@ -726,8 +729,11 @@ int eph_server_result_ensemble(eph_server *srv, int64_t job, eph_mass *masses, d
 int eph_server_snapshot_ensemble(eph_server *srv, int64_t job, eph_mass *masses, double *t_eph, int64_t *steps_done) {
     if (!srv) return fail(EPH_EINVAL, "null server");
     Server *s = S(srv);
-    // between ticks of the ensemble loop the resident batch holds whole quanta
-    std::lock_guard<std::mutex> tk(s->tick_mu[EPH_JOB_ENSEMBLE]);
+    // between ticks of the ensemble loop the resident batch holds whole
+    // quanta; the worker lets a waiting snapshot take the lock between ticks
+    s->snap_waiting.fetch_add(1);
+    std::unique_lock<std::mutex> tk(s->tick_mu[EPH_JOB_ENSEMBLE]);
+    s->snap_waiting.fetch_sub(1);
     SJob *j = nullptr;
     {
         std::lock_guard<std::mutex> lk(s->mu);

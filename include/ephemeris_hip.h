@@ -261,8 +261,8 @@ int eph_server_job_info(eph_server *srv, int64_t job, eph_job_info *info);
 int eph_server_result_ensemble(eph_server *srv, int64_t job, eph_mass *masses, double *t_eph);
 int eph_server_result_particles(eph_server *srv, int64_t job, double *r, double *v, double *extrema);
 /* State of an ensemble job at the last whole quantum (queued: its input;
- * running: read from the resident batch between ticks; done: its result),
- * with the steps done so far.  A caller writes it as the reference's binary
+ * running: read from the resident batch between ticks, waiting at most for
+ * the tick in progress; done: its result), with the steps done so far.  A caller writes it as the reference's binary
  * checkpoint (eph_msystem_create, eph_msystem_analyse with reconstruct = 1,
  * eph_msystem_save_checkpoint; one file per member) and resumes from it by
  * loading the file and submitting the rest of the span.  No counterpart in

@@ -208,3 +208,40 @@ def test_snapshot_checkpoint_resume(setup, tmp_path):
     print(f"resumed member: {n0} bodies, mismatching fields {bad_f}", flush=True)
     assert not bad_f and bits_equal(tr[0], tf[0])
     srv.close()
+
+
+def test_snapshots_while_the_worker_runs(setup):
+    """Snapshots taken from another thread while the worker ticks: each
+    holds whole quanta (steps done a multiple of the quantum, never
+    decreasing) and equals the job run alone for that many steps."""
+    import time
+    cfg, members, r, v = setup
+    print("\n[case] server snapshot from the caller's thread while the worker runs", flush=True)
+    srv = Server(_field(cfg), ens_dt=ENS_DT, ens_quantum=ENS_Q, part_dt=PART_DT, part_quantum=PART_Q,
+                 part_max_steps=PART_MAX)
+    job = _job(members, [2])
+    total = 40 * ENS_Q
+    jid = srv.submit_ensemble(*job, n_steps=total)
+    srv.start()
+    snaps = {}
+    t_end = time.monotonic() + 120
+    while time.monotonic() < t_end:
+        m, t, k = srv.snapshot_ensemble(jid)
+        snaps.setdefault(k, (m, t))
+        if k >= total or srv.info(jid)["status"] in (DONE, FAILED):
+            break
+        time.sleep(0.002)
+    srv.wait(120)
+    srv.stop()
+    ks = list(snaps)
+    print("steps seen in snapshots:", ks, flush=True)
+    assert srv.info(jid)["status"] == DONE
+    assert ks == sorted(ks) and all(k % ENS_Q == 0 for k in ks)
+    mid = [k for k in ks if 0 < k < total]
+    for k in mid[:2]:
+        ma, ta = _alone_ensemble(job, k)
+        assert bits_equal(snaps[k][0], ma) and bits_equal(snaps[k][1], ta), k
+    mf, tf = srv.result_ensemble(jid)
+    m3, t3, k3 = srv.snapshot_ensemble(jid)
+    assert k3 == total and bits_equal(m3, mf) and bits_equal(t3, tf)
+    srv.close()
