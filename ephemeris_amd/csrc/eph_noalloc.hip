@@ -27,15 +27,18 @@ constexpr size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 struct NaPlan {
     SysMeta sys;
     NPItem items[kNaMaxItems];
-    int4 blk_light[kNaMaxBlocks], blk_damped[kNaMaxBlocks];
+    NPItem sitems[kNaMaxItems];                                   // single-thread order: each model's field rows
+    int32_t srow_gp[kNaMaxBodies], srow_ring[kNaMaxBodies];
     int2 blk_main[kNaMaxBlocks], blk_recv[kNaMaxBlocks];
-    int32_t perm[kNaMaxBodies], np_gp[kNaMaxBodies], np_ring[kNaMaxBodies], bsys[kNaMaxBodies];
+    int2 work[kNaMaxBodies];                                      // (system 0, body) of every body
+    int32_t np_gp[kNaMaxBodies], np_ring[kNaMaxBodies], bsys[kNaMaxBodies];
     double t0;
 };
 constexpr size_t kOffF = 0;                                                            // fields [0, 121) x 1024
 constexpr size_t kOffAn = align256(kOffF + sizeof(double) * kMassFieldEnd * kNaMaxBodies);
 constexpr size_t kOffPart = align256(kOffAn + sizeof(double) * 3 * kNaMaxItems * kNaMaxBodies);
-constexpr size_t kOffPlan = align256(kOffPart + sizeof(double) * 6 * kNaMaxItems * kNaMaxItems);
+constexpr size_t kOffModels = align256(kOffPart + sizeof(double) * 6 * kNaMaxItems * kNaMaxItems);
+constexpr size_t kOffPlan = align256(kOffModels + sizeof(int64_t) * 2 * kNaMaxBodies);
 constexpr size_t kScratchBytes = align256(kOffPlan + sizeof(NaPlan));     // about 2.8 MB
 
 }  // namespace
@@ -79,6 +82,14 @@ __global__ void k_na_scatter(const double *__restrict__ F, int64_t nbp, int n, c
     out[g].ring = ring;
 }
 
+// the model indices (gp, ring) of every body, contiguous, for one host read
+__global__ void k_na_models(const eph_mass *__restrict__ in, int n, int64_t *__restrict__ out) {
+    const int g = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (g >= n) return;
+    out[2 * g] = in[g].gp;
+    out[2 * g + 1] = in[g].ring;
+}
+
 // true when p is device memory of the current device (the kernels read it)
 bool on_device(const void *p) {
     hipPointerAttribute_t a;
@@ -87,14 +98,6 @@ bool on_device(const void *p) {
         return false;
     }
     return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
-}
-
-// one field of every body, device AoS -> host array
-template <typename T>
-int read_member(const eph_mass *d, int64_t n, size_t offset, T *host) {
-    HIP_CHECK(hipMemcpy2D(host, sizeof(T), (const char *)d + offset, sizeof(eph_mass), sizeof(T), (size_t)n,
-                          hipMemcpyDeviceToHost));
-    return EPH_OK;
 }
 
 }  // namespace
@@ -141,40 +144,45 @@ extern "C" int eph_accel_noalloc(const eph_mass *masses, int64_t n, const eph_ge
     char *base = (char *)g_na_ptr[dev];
 
     // what the plan needs of each body, read from device memory
-    int64_t gp[kNaMaxBodies], rg[kNaMaxBodies];
-    double k2[kNaMaxBodies], td[kNaMaxBodies];
-    int rc;
-    if ((rc = read_member(masses, n, offsetof(eph_mass, gp), gp))) return rc;
-    if ((rc = read_member(masses, n, offsetof(eph_mass, ring), rg))) return rc;
-    if ((rc = read_member(masses, n, offsetof(eph_mass, k2), k2))) return rc;
-    if ((rc = read_member(masses, n, offsetof(eph_mass, tide_delay), td))) return rc;
+    int64_t models[2 * kNaMaxBodies];
+    hipLaunchKernelGGL(k_na_models, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, 0, masses, (int)n,
+                       (int64_t *)(base + kOffModels));
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpy(models, base + kOffModels, sizeof(int64_t) * 2 * n, hipMemcpyDeviceToHost));
+    const int64_t *gp = models;                           // gp[2 i], ring[2 i + 1]
 
     static thread_local NaPlan plan;                      // host image of the plan (no allocation)
     memset(&plan, 0, sizeof plan);
     int64_t nnp = 0;
     for (int64_t i = 0; i < n; ++i) {
-        if (gp[i] >= n_gp || rg[i] >= n_ring || (gp[i] >= 0 && !gps) || (rg[i] >= 0 && !rings))
+        const int64_t g = gp[2 * i], r = gp[2 * i + 1];
+        if (g >= n_gp || r >= n_ring || (g >= 0 && !gps) || (r >= 0 && !rings))
             return fail(EPH_EINVAL, "eph_accel_noalloc: model index out of range");
-        plan.np_gp[i] = gp[i] >= 0 ? (int32_t)gp[i] : -1;
-        plan.np_ring[i] = rg[i] >= 0 ? (int32_t)rg[i] : -1;
-        nnp += (gp[i] >= 0) + (rg[i] >= 0);
+        plan.np_gp[i] = g >= 0 ? (int32_t)g : -1;
+        plan.np_ring[i] = r >= 0 ? (int32_t)r : -1;
+        nnp += (g >= 0) + (r >= 0);
     }
     if (nnp > kNaMaxItems) return fail(EPH_EINVAL, "eph_accel_noalloc: more than 64 geopotential and ring models");
-    // slots: bodies with constant-time-lag tides first (deform kernels take one class per block)
-    int nd = 0, k = 0;
-    for (int pass = 0; pass < 2; ++pass)
-        for (int64_t i = 0; i < n; ++i) {
-            const bool damped = k2[i] != 0 && td[i] != 0;
-            if (damped == (pass == 0)) plan.perm[k++] = (int32_t)i;
-            if (pass == 0 && damped) ++nd;
-        }
-    int n_damped = 0, n_light = 0, n_main = 0, n_recv = 0;
-    for (int q = 0; q < nd; q += kBS) plan.blk_damped[n_damped++] = make_int4(0, q, std::min(kBS, nd - q), 0);
-    for (int q = nd; q < (int)n; q += kBS) plan.blk_light[n_light++] = make_int4(0, q, std::min<int>(kBS, (int)n - q), 0);
+    int n_main = 0, n_recv = 0;
     for (int q = 0; q < (int)n; q += kBS) plan.blk_main[n_main++] = make_int2(0, q);
+    for (int i = 0; i < (int)n; ++i) plan.work[i] = make_int2(0, i);
     // summation order and the non-point rounds (msystem::accel, mass.cpp:226-307)
     const int64_t n_task = std::min<int64_t>(n, (n * n + 2303) / 2304);
     const bool serial = summation == EPH_SUMMATION_SERIAL || n_task <= 1;
+    int n_sitems = 0;
+    if (serial) {                                         // each model's field at every partner, formed first
+        int32_t san_rows = 0;
+        for (int64_t i = 0; i < n; ++i) {
+            plan.srow_gp[i] = plan.srow_ring[i] = -1;
+            for (int kind = 0; kind < 2; ++kind) {
+                const int32_t model = kind == 0 ? plan.np_gp[i] : plan.np_ring[i];
+                if (model < 0) continue;
+                plan.sitems[n_sitems++] = NPItem{(int32_t)i, kind, model, 0, 0, san_rows};
+                (kind == 0 ? plan.srow_gp : plan.srow_ring)[i] = san_rows;
+                san_rows += (int32_t)n;
+            }
+        }
+    }
     SysMeta &sm = plan.sys;
     sm.off = 0;
     sm.n = (int32_t)n;
@@ -215,19 +223,32 @@ extern "C" int eph_accel_noalloc(const eph_mass *masses, int64_t n, const eph_ge
     v.np_gp = dp->np_gp;
     v.np_ring = dp->np_ring;
     v.bsys = dp->bsys;
-    v.perm = dp->perm;
-    v.srow_gp = nullptr;
-    v.srow_ring = nullptr;
-    v.san = nullptr;
+    v.perm = nullptr;                                     // (read only by the four-lane and one-thread-per-body deform kernels)
+    v.srow_gp = dp->srow_gp;
+    v.srow_ring = dp->srow_ring;
+    v.san = n_sitems ? (const double *)(base + kOffAn) : nullptr;      // rows of at most 64 models x n partners
     v.an = (double *)(base + kOffAn);
     v.part = (double *)(base + kOffPart);
     v.t0 = &dp->t0;
     const hipStream_t st = 0;
     hipLaunchKernelGGL(k_na_gather, dim3((unsigned)((v.nbp + 63) / 64)), dim3(64), 0, st, masses, (int)n, v.nbp, v.F);
-    launch_deform(v, dp->blk_light, n_light, dp->blk_damped, n_damped, 1, st);
-    if (serial) launch_main_serial(v, dp->blk_main, n_main, st);
-    else launch_main_pool(v, dp->blk_main, n_main, st);
-    if (!serial && nnp) launch_np(v, (int)nnp, (int)n, maxT, dp->blk_recv, n_recv, true, st);
+    // the kernels of the resident path for a batch of one system: one
+    // workgroup per body (the item fields in extra workgroups of the deform
+    // launch), then the main force in the order asked for and the non-point pass
+    const int rows = serial ? 0 : (int)nnp;
+    launch_deform_wpb(v, dp->work, (int)n, (int)n, 1, rows, 0, st);
+    if (serial) {
+        if (n_sitems) {
+            View vs = v;
+            vs.items = dp->sitems;
+            vs.an = (double *)(base + kOffAn);
+            launch_np_field(vs, n_sitems, (int)n, st);
+        }
+        launch_main_serial(v, dp->blk_main, n_main, st);
+    } else {
+        launch_main_wpb(v, dp->work, (int)n, (int)n, false, 0, st);
+        if (nnp) launch_np(v, (int)nnp, (int)n, maxT, dp->blk_recv, n_recv, false, st);
+    }
     hipLaunchKernelGGL(k_na_scatter, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, v.F, v.nbp, (int)n, masses, out);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(st));
