@@ -125,8 +125,9 @@ int eph_integrate(eph_system *sys, double dt, int64_t n_step);
 int eph_integrate_ex(eph_system *sys, double dt, int64_t n_step, int summation);
 
 /* Allocation-free force evaluation.  No counterpart in the reference: for
- * callers that cannot allocate device memory (code under HIP graph capture,
- * tight serving loops, harnesses that forbid hipMalloc).  One msystem::accel
+ * callers that cannot allocate device memory (tight serving loops,
+ * harnesses that forbid hipMalloc; it synchronises with the device, so it is
+ * not a call to make under stream capture).  One msystem::accel
  * (parallel_option 0 with summation EPH_SUMMATION_POOL, -1 with
  * EPH_SUMMATION_SERIAL) of the n bodies at `masses` with their models, all in
  * device memory, written to `out` (device memory, n bodies; may be `masses`);
