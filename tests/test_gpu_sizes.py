@@ -12,13 +12,13 @@
   deform kernel's cached pair geometry, so its non-point pass runs in its
   own kernels, its constant-time-lag bodies re-form the pair geometry on
   every spin pass, and its main force takes four rounds of 256 partners.
+* A batch of all of them together: each member ends with the bits it has
+  alone.
 * Empty inputs are rejected.
 
 Force evaluations must equal the oracle's bit for bit (ring-carrying
 outputs to 1e-13 relative: device libm), and a few RK12 steps must stay
 within 1e-15 in position and 1e-13 in velocity."""
-import tempfile
-
 import numpy as np
 import pytest
 
@@ -116,3 +116,40 @@ def test_empty_inputs_are_rejected():
         GpuSystems(m[:0], np.zeros(0, np.int64))
     with pytest.raises(_lib.EphError):
         GpuSystems(m, np.array([25, 0]))
+
+
+def test_batch_of_extreme_sizes_equals_members_alone(tmp_path):
+    """One batch holding systems of 1, 2, 3 and 804 bodies (single-thread and
+    thread-pool orders side by side): every member's bits after three steps
+    equal the member integrated alone."""
+    print("\n[case] batch of 1-, 2-, 3- and 804-body systems vs each alone, 3 steps", flush=True)
+    full = po.OSystem.load(CONFIGS["full504"], pool_at_load=False)
+    ms = full.masses
+    host = int(np.nonzero(ms["ring"] >= 0)[0][0])
+    large = po.OSystem.load(gen.write_system(gen.add_asteroids(gen.make_full(), 300), tmp_path, "Large"),
+                            pool_at_load=False)
+    members = [full.subset([0]), full.subset([0, host]), full.subset([0, host, host + 1]), large]
+    parts, sys_n, gps_all, rings_all = [], [], [], []
+    ngp = nring = 0
+    for o in members:
+        m = o.masses.copy()
+        g, r = o.models()
+        m["gp"] = np.where(m["gp"] >= 0, m["gp"] + ngp, -1)
+        m["ring"] = np.where(m["ring"] >= 0, m["ring"] + nring, -1)
+        ngp += len(g)
+        nring += len(r)
+        parts.append(m), sys_n.append(len(m)), gps_all.append(g), rings_all.append(r)
+    batch = GpuSystems(np.concatenate(parts), np.array(sys_n), np.concatenate(gps_all), np.concatenate(rings_all))
+    batch.integrate(300.0, 3)
+    mb, _ = batch.download()
+    off = 0
+    for o, n in zip(members, sys_n):
+        g, r = o.models()
+        alone = GpuSystems(o.masses.copy(), None, g, r)
+        alone.integrate(300.0, 3)
+        ma, _ = alone.download()
+        mine = mb[off:off + n].copy()
+        bad = [f for f in MASS_DTYPE.names if f not in ("gp", "ring") and not bits_equal(mine[f], ma[f])]
+        print(f"{n} bodies: fields differing from the member alone {bad}", flush=True)
+        assert not bad
+        off += n
