@@ -203,6 +203,11 @@ def test_snapshot_checkpoint_resume(setup, tmp_path):
     m3, t3, k3 = srv.snapshot_ensemble(jid)
     mf, tf = srv.result_ensemble(jid)
     assert k3 == 3 * ENS_Q and bits_equal(m3, mf) and bits_equal(t3, tf)
+    pj = srv.submit_particles(r[:10], v[:10], PART_Q)
+    for bad_id in (pj, 10 ** 9):                            # a particle job, an unknown id
+        srv._shape.setdefault(bad_id, srv._shape[jid])
+        with pytest.raises(Exception):
+            srv.snapshot_ensemble(bad_id)
     keep = [f for f in mf.dtype.names if f not in ("gp", "ring")]
     bad_f = [f for f in keep if not bits_equal(mr[f], mf[:n0][f])]
     print(f"resumed member: {n0} bodies, mismatching fields {bad_f}", flush=True)
