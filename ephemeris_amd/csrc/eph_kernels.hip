@@ -1580,14 +1580,13 @@ __device__ __forceinline__ void stage_rotational(const Acc &A, int g, int k, dou
     A.stv(F_W, g, w);
 }
 
-__device__ __forceinline__ void stage_body_inl(const View &d, const Acc &A, int g, int k, double dt, double dt_k, double tadd) {
+// the persistent small-system kernel keeps the two halves of the stage
+// update out of line and runs them on different threads
+__device__ __noinline__ void stage_translational_ool(const View &d, const Acc &A, int g, int k, double dt, double tadd) {
     stage_translational(d, A, g, k, dt, tadd);
-    stage_rotational(A, g, k, dt, dt_k);
 }
-
-// the persistent small-system kernel keeps the stage update out of line
-__device__ __noinline__ void stage_body(const View &d, const Acc &A, int g, int k, double dt, double dt_k, double tadd) {
-    stage_body_inl(d, A, g, k, dt, dt_k, tadd);
+__device__ __noinline__ void stage_rotational_ool(const Acc &A, int g, int k, double dt, double dt_k) {
+    stage_rotational(A, g, k, dt, dt_k);
 }
 
 // istep != nullptr: the step index comes from device memory (replayed graphs)
@@ -1723,11 +1722,13 @@ __device__ void main_serial_body_tile(const View &d, const Acc &A, int gi, int i
 }
 
 // One workgroup integrates one system (N <= kSmallMax bodies, single-thread
-// summation order) for n_steps RK12 steps with no launch per stage.  With
+// summation order) for n_steps RK12 steps with no launch per stage: thread
+// t < N takes body t, and threads kSmallMax + t the rotational half of the
+// stage update of body t (the two halves touch different fields).  With
 // `rec`, the partner data (JMain) of every body after each stage's deform is
 // recorded: entry e = 25*step + (k-1) of the launch, workgroup b, body t at
 // rec[(e*nrec_sys + b)*kSmallMax + t].
-__global__ __launch_bounds__(kSmallMax) void k_small(View d, const int *__restrict__ sys_ids, int n_steps,
+__global__ __launch_bounds__(2 * kSmallMax) void k_small(View d, const int *__restrict__ sys_ids, int n_steps,
                                                      double dt, DtK dtk, int64_t i_step0, JMain *rec, int nrec_sys) {
     const Acc A{d.F, d.nbp};
     const int s = sys_ids[blockIdx.x];
@@ -1739,7 +1740,8 @@ __global__ __launch_bounds__(kSmallMax) void k_small(View d, const int *__restri
     for (int step = 0; step < n_steps; ++step) {
         for (int k = 1; k <= 25; ++k) {
             const double tadd = dt * (dtk.v[k] + (double)(i_step0 + step));
-            if (valid) stage_body(d, A, off + t, k, dt, dtk.v[k], tadd);
+            if (valid) stage_translational_ool(d, A, off + t, k, dt, tadd);
+            else if (t >= kSmallMax && t - kSmallMax < n) stage_rotational_ool(A, off + t - kSmallMax, k, dt, dtk.v[k]);
             __syncthreads();
             if (valid) load_jdeform(A, tdf[t], off + t);
             __syncthreads();
@@ -2048,7 +2050,7 @@ void launch_step_count(int64_t *istep, hipStream_t st) {
 }
 void launch_small(const View &d, const int *sys_ids, int nsys, int n_steps, double dt, const DtK &dtk,
                   int64_t i_step0, JMain *rec, hipStream_t st) {
-    if (nsys) hipLaunchKernelGGL(k_small, dim3(nsys), dim3(kSmallMax), 0, st, d, sys_ids, n_steps, dt, dtk, i_step0, rec, nsys);
+    if (nsys) hipLaunchKernelGGL(k_small, dim3(nsys), dim3(2 * kSmallMax), 0, st, d, sys_ids, n_steps, dt, dtk, i_step0, rec, nsys);
 }
 void launch_extrema(const View &d, int invert, hipStream_t st) {
     hipLaunchKernelGGL(k_extrema, dim3((d.nb + 255) / 256), dim3(256), 0, st, d, invert);
