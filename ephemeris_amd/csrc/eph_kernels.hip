@@ -1282,8 +1282,14 @@ __device__ __forceinline__ v3 san_at(const View &d, int row, int jl) {
     return v3{a[0], a[1], a[2]};
 }
 
-__device__ __forceinline__ void serial_reaction(const View &d, const Acc &A, int gp_, int gi, int il, const JMain &T,
-                                                const IMain &I, v3 &da, bool rows) {
+// The terms that outer body p adds to partner i's daccel when it visits i
+// (ROTATIONAL_TIDAL_DEFORMATION's reaction, then APPLY_NONPOINT_FORCE of
+// p's geopotential and ring): -GM_p dg, +GM_p field_gp, +GM_p field_ring,
+// in that order; a model p does not have gives +0 (and is not added).
+struct SerialReact { v3 t[3]; bool gp, ring; };
+__device__ __forceinline__ SerialReact serial_reaction_terms(const View &d, const Acc &A, int gp_, int il, const JMain &T,
+                                                             const IMain &I, bool rows) {
+    SerialReact o;
     // outer body p visiting partner i: r = r_i - r_p
     const ddv3 rp{dd{T.rxh, T.rxl}, dd{T.ryh, T.ryl}, dd{T.rzh, T.rzl}};
     PairGeo p;
@@ -1293,12 +1299,24 @@ __device__ __forceinline__ void serial_reaction(const View &d, const Acc &A, int
     p.rr3 = p.rr * p.rr2;
     const m3 Cp{v3{T.C[0], T.C[1], T.C[2]}, v3{T.C[3], T.C[4], T.C[5]}, v3{T.C[6], T.C[7], T.C[8]}};
     const v3 dg = (p.rr3 * p.rr2 * T.R2) * quad_field(Cp, p);
-    da -= T.GM * dg;
+    o.t[0] = -(T.GM * dg);                                 // da -= GM dg
     const int gpi = d.np_gp[gp_], rgi = d.np_ring[gp_];
+    o.gp = gpi >= 0;
+    o.ring = rgi >= 0;
+    o.t[1] = o.t[2] = v3{0, 0, 0};
     // the field of p's models at i (r = r_i - r_p): from the rows formed
     // beforehand (rows), or evaluated here
-    if (gpi >= 0) da += T.GM * (rows ? san_at(d, d.srow_gp[gp_], il) : nonpoint_field_call(d, A, gp_, 0, gpi, p.r));
-    if (rgi >= 0) da += T.GM * (rows ? san_at(d, d.srow_ring[gp_], il) : nonpoint_field_call(d, A, gp_, 1, rgi, p.r));
+    if (o.gp) o.t[1] = T.GM * (rows ? san_at(d, d.srow_gp[gp_], il) : nonpoint_field_call(d, A, gp_, 0, gpi, p.r));
+    if (o.ring) o.t[2] = T.GM * (rows ? san_at(d, d.srow_ring[gp_], il) : nonpoint_field_call(d, A, gp_, 1, rgi, p.r));
+    return o;
+}
+
+__device__ __forceinline__ void serial_reaction(const View &d, const Acc &A, int gp_, int gi, int il, const JMain &T,
+                                                const IMain &I, v3 &da, bool rows) {
+    const SerialReact o = serial_reaction_terms(d, A, gp_, il, T, I, rows);
+    da += o.t[0];
+    if (o.gp) da += o.t[1];
+    if (o.ring) da += o.t[2];
 }
 
 template <int BS>
@@ -1686,39 +1704,120 @@ __device__ void deform_body_tile(const Acc &A, int gi, int il, int n, const JDef
     deform_write(A, gi, 1, vi, phi, nacc, C, GI, w);
 }
 
-__device__ void main_serial_body_tile(const View &d, const Acc &A, int gi, int il, int n, const JMain *tile) {
-    const int off = gi - il;
+// Main force of a small system in the single-thread order, by the whole
+// workgroup: each thread forms every term of one ordered pair (i, q) -- the
+// own-loop terms of i's partner q and the reaction terms of outer body q on
+// i -- into LDS; then ten threads per body add them in the order of the
+// sequential loop (mass.cpp:329-374): daccel takes the reactions of q < i,
+// the own-loop terms of every q, the reactions of q > i; dtorque and gaccel
+// the own-loop terms; the running maxima go over the partners.  A body's
+// pairs fall in one round, so its sums are never split.  Terms of a model
+// the body does not have are +0, an exact identity in these sums.
+constexpr int kSmallPairs = 2 * kSmallMax;            // pairs per round (one per thread)
+constexpr int kSmallTerm = 44;                         // doubles per pair
+// pair slots: 0-2 G; 3-20 daccel own terms (GM dg, -LT own spin, +LT
+// partner spin, radiation, -GM field_gp, -GM field_ring); 21-32 dtorque
+// own terms (GM r x dg, -LT torque, -GM r x field_gp, -GM r x field_ring);
+// 33-41 daccel reaction terms (-GM_q dg_q, +GM_q field_gp, +GM_q field_ring);
+// 42 1/r, 43 GM/r^3
+__device__ void small_pair_terms(const View &d, const Acc &A, int off, int il, int q, const JMain &T, double *o) {
+    const int gi = off + il;
     const IMain I = load_imain(A, gi);
     const int my_gp = d.np_gp[gi], my_ring = d.np_ring[gi];
-    v3 ga{0, 0, 0}, da{0, 0, 0}, dq{0, 0, 0};
-    double mind = A.ld(F_MIND, gi), maxi = A.ld(F_MAXI, gi);
-    for (int q = 0; q < il; ++q) serial_reaction(d, A, off + q, gi, il, tile[q], I, da, false);
-    for (int q = 0; q < n; ++q) {
-        if (q == il) continue;
-        const JMain &T = tile[q];
-        const PairGeo p = relativity(I, T, ga);
-        mind = checked_max_into(mind, p.rr);
-        maxi = checked_max_into(maxi, p.tp_dg);
-        const v3 dg = (p.rr3 * p.rr2 * I.R2) * quad_field(I.C, p);
-        da += T.GM * dg;
-        dq += T.GM * cross(p.r, dg);
-        lt_rad(I, T, p, da, dq);
-        if (my_gp >= 0) {
-            const v3 an = nonpoint_field_call(d, A, gi, 0, my_gp, p.r);
-            da -= T.GM * an;
-            dq -= T.GM * cross(p.r, an);
-        }
-        if (my_ring >= 0) {
-            const v3 an = nonpoint_field_call(d, A, gi, 1, my_ring, p.r);
-            da -= T.GM * an;
-            dq -= T.GM * cross(p.r, an);
+    v3 unused{0, 0, 0};
+    const PairGeo p = relativity(I, T, unused);
+    v3 ta[6], tq[4];
+    const v3 dg = (p.rr3 * p.rr2 * I.R2) * quad_field(I.C, p);
+    ta[0] = T.GM * dg;
+    tq[0] = T.GM * cross(p.r, dg);
+    // LENSE_THIRRING and RADIATION_PRESSURE as lt_rad forms them
+    const double rcr32 = 2 * T.GM / kC * p.rr3;
+    const v3 GLb = cross(I.GL, p.b);
+    ta[1] = -(rcr32 * (GLb - (3 * dot(GLb, p.r) * p.rr2) * p.r));
+    tq[1] = -(rcr32 * cross(I.GL, cross(p.r, p.b)));
+    const v3 GLj{T.GLx, T.GLy, T.GLz};
+    ta[2] = cross(rcr32 * (GLj - (3 * dot(GLj, p.r) * p.rr2) * p.r), p.b);
+    ta[3] = (T.lum * I.rR2G4c * p.rr3 / I.GM) * p.r;
+    ta[4] = ta[5] = tq[2] = tq[3] = v3{0, 0, 0};
+    if (my_gp >= 0) {
+        const v3 an = nonpoint_field_call(d, A, gi, 0, my_gp, p.r);
+        ta[4] = -(T.GM * an);
+        tq[2] = -(T.GM * cross(p.r, an));
+    }
+    if (my_ring >= 0) {
+        const v3 an = nonpoint_field_call(d, A, gi, 1, my_ring, p.r);
+        ta[5] = -(T.GM * an);
+        tq[3] = -(T.GM * cross(p.r, an));
+    }
+    const SerialReact r = serial_reaction_terms(d, A, off + q, il, T, I, false);
+    o[0] = p.G.x; o[1] = p.G.y; o[2] = p.G.z;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) { o[3 + 3 * k] = ta[k].x; o[4 + 3 * k] = ta[k].y; o[5 + 3 * k] = ta[k].z; }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) { o[21 + 3 * k] = tq[k].x; o[22 + 3 * k] = tq[k].y; o[23 + 3 * k] = tq[k].z; }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { o[33 + 3 * k] = r.t[k].x; o[34 + 3 * k] = r.t[k].y; o[35 + 3 * k] = r.t[k].z; }
+    o[42] = p.rr;
+    o[43] = p.tp_dg;
+}
+
+// called by every thread of the workgroup (it synchronises)
+__device__ void main_serial_small(const View &d, const Acc &A, int off, int n, const JMain *tile, double *terms,
+                                  double *sums) {
+    const int tid = (int)threadIdx.x;
+    if (n > 1) {
+        const int m = n - 1;                                // pairs per body
+        const int bpr = min(12, kSmallPairs / m);           // bodies per round (10 sum threads each)
+        for (int b0 = 0; b0 < n; b0 += bpr) {
+            const int nb = min(bpr, n - b0);
+            if (tid < nb * m) {
+                const int il = b0 + tid / m, qq = tid - (tid / m) * m;
+                const int q = qq < il ? qq : qq + 1;
+                small_pair_terms(d, A, off, il, q, tile[q], terms + tid * kSmallTerm);
+            }
+            __syncthreads();
+            if (tid < 10 * nb) {
+                const int il = b0 + tid / 10, c = tid - (tid / 10) * 10;
+                const double *P = terms + (il - b0) * m * kSmallTerm;   // this body's pairs, q ascending
+                double acc = 0;
+                if (c < 3) {                                // daccel
+                    for (int qq = 0; qq < il; ++qq)
+                        for (int k = 0; k < 3; ++k) acc += P[qq * kSmallTerm + 33 + 3 * k + c];
+                    for (int qq = 0; qq < m; ++qq)
+                        for (int k = 0; k < 6; ++k) acc += P[qq * kSmallTerm + 3 + 3 * k + c];
+                    for (int qq = il; qq < m; ++qq)
+                        for (int k = 0; k < 3; ++k) acc += P[qq * kSmallTerm + 33 + 3 * k + c];
+                } else if (c < 6) {                         // dtorque
+                    for (int qq = 0; qq < m; ++qq)
+                        for (int k = 0; k < 4; ++k) acc += P[qq * kSmallTerm + 21 + 3 * k + (c - 3)];
+                } else if (c < 9) {                         // gaccel
+                    for (int qq = 0; qq < m; ++qq) acc += P[qq * kSmallTerm + (c - 6)];
+                } else {                                    // running maxima
+                    const int gi = off + il;
+                    double mind = A.ld(F_MIND, gi), maxi = A.ld(F_MAXI, gi);
+                    for (int qq = 0; qq < m; ++qq) {
+                        mind = checked_max_into(mind, P[qq * kSmallTerm + 42]);
+                        maxi = checked_max_into(maxi, P[qq * kSmallTerm + 43]);
+                    }
+                    sums[il * 11 + 10] = maxi;
+                    acc = mind;
+                }
+                sums[il * 11 + c] = acc;
+            }
+            __syncthreads();
         }
     }
-    for (int q = il + 1; q < n; ++q) serial_reaction(d, A, off + q, gi, il, tile[q], I, da, false);
-    A.stv(F_GACC, gi, ga);
-    A.st(F_MIND, gi, mind);
-    A.st(F_MAXI, gi, maxi);
-    finalize_body(d, A, gi, da, dq);
+    if (tid < n) {
+        const int gi = off + tid;
+        const double *S = sums + tid * 11;
+        const bool any = n > 1;
+        A.stv(F_GACC, gi, any ? v3{S[6], S[7], S[8]} : v3{0, 0, 0});
+        if (any) {
+            A.st(F_MIND, gi, S[9]);
+            A.st(F_MAXI, gi, S[10]);
+        }
+        finalize_body(d, A, gi, any ? v3{S[0], S[1], S[2]} : v3{0, 0, 0}, any ? v3{S[3], S[4], S[5]} : v3{0, 0, 0});
+    }
 }
 
 // One workgroup integrates one system (N <= kSmallMax bodies, single-thread
@@ -1737,6 +1836,8 @@ __global__ __launch_bounds__(2 * kSmallMax) void k_small(View d, const int *__re
     const bool valid = t < n;
     __shared__ JDeform tdf[kSmallMax];
     __shared__ JMain tmn[kSmallMax];
+    __shared__ double terms[kSmallPairs * kSmallTerm];
+    __shared__ double sums[kSmallMax * 11];
     for (int step = 0; step < n_steps; ++step) {
         for (int k = 1; k <= 25; ++k) {
             const double tadd = dt * (dtk.v[k] + (double)(i_step0 + step));
@@ -1749,7 +1850,7 @@ __global__ __launch_bounds__(2 * kSmallMax) void k_small(View d, const int *__re
             __syncthreads();
             if (valid) load_jmain(A, tmn[t], off + t);
             __syncthreads();
-            if (valid) main_serial_body_tile(d, A, off + t, t, n, tmn);
+            main_serial_small(d, A, off, n, tmn, terms, sums);
             if (rec && valid) {
                 const int64_t e = 25 * (int64_t)step + (k - 1);
                 rec[(e * nrec_sys + blockIdx.x) * kSmallMax + t] = tmn[t];
