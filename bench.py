@@ -301,7 +301,8 @@ def main():
         return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported (each system, clone or particle "
-              "set fits one MI355X; see DESIGN.md for how the work would shard)", file=sys.stderr)
+              "set fits one MI355X; clones, jobs and particles would be split across GPUs by index with no "
+              "exchange between them)", file=sys.stderr)
         sys.exit(2)
 
     from ephemeris_amd.system import GpuSystems
