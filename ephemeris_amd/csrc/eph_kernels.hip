@@ -1713,7 +1713,6 @@ __device__ void deform_body_tile(const Acc &A, int gi, int il, int n, const JDef
 // the own-loop terms; the running maxima go over the partners.  A body's
 // pairs fall in one round, so its sums are never split.  Terms of a model
 // the body does not have are +0, an exact identity in these sums.
-constexpr int kSmallPairs = 2 * kSmallMax;            // pairs per round (one per thread)
 constexpr int kSmallTerm = 44;                         // doubles per pair
 // pair slots: 0-2 G; 3-20 daccel own terms (GM dg, -LT own spin, +LT
 // partner spin, radiation, -GM field_gp, -GM field_ring); 21-32 dtorque
@@ -1761,13 +1760,14 @@ __device__ void small_pair_terms(const View &d, const Acc &A, int off, int il, i
     o[43] = p.tp_dg;
 }
 
-// called by every thread of the workgroup (it synchronises)
+// called by every thread of the workgroup of NT threads (it synchronises)
+template <int NT>
 __device__ void main_serial_small(const View &d, const Acc &A, int off, int n, const JMain *tile, double *terms,
                                   double *sums) {
     const int tid = (int)threadIdx.x;
     if (n > 1) {
         const int m = n - 1;                                // pairs per body
-        const int bpr = min(12, kSmallPairs / m);           // bodies per round (10 sum threads each)
+        const int bpr = min(12, NT / m);                    // bodies per round (10 sum threads each)
         for (int b0 = 0; b0 < n; b0 += bpr) {
             const int nb = min(bpr, n - b0);
             if (tid < nb * m) {
@@ -1827,7 +1827,8 @@ __device__ void main_serial_small(const View &d, const Acc &A, int off, int n, c
 // `rec`, the partner data (JMain) of every body after each stage's deform is
 // recorded: entry e = 25*step + (k-1) of the launch, workgroup b, body t at
 // rec[(e*nrec_sys + b)*kSmallMax + t].
-__global__ __launch_bounds__(2 * kSmallMax) void k_small(View d, const int *__restrict__ sys_ids, int n_steps,
+template <int NT>
+__global__ __launch_bounds__(NT) void k_small(View d, const int *__restrict__ sys_ids, int n_steps,
                                                      double dt, DtK dtk, int64_t i_step0, JMain *rec, int nrec_sys) {
     const Acc A{d.F, d.nbp};
     const int s = sys_ids[blockIdx.x];
@@ -1836,7 +1837,7 @@ __global__ __launch_bounds__(2 * kSmallMax) void k_small(View d, const int *__re
     const bool valid = t < n;
     __shared__ JDeform tdf[kSmallMax];
     __shared__ JMain tmn[kSmallMax];
-    __shared__ double terms[kSmallPairs * kSmallTerm];
+    __shared__ double terms[NT * kSmallTerm];
     __shared__ double sums[kSmallMax * 11];
     for (int step = 0; step < n_steps; ++step) {
         for (int k = 1; k <= 25; ++k) {
@@ -1850,7 +1851,7 @@ __global__ __launch_bounds__(2 * kSmallMax) void k_small(View d, const int *__re
             __syncthreads();
             if (valid) load_jmain(A, tmn[t], off + t);
             __syncthreads();
-            main_serial_small(d, A, off, n, tmn, terms, sums);
+            main_serial_small<NT>(d, A, off, n, tmn, terms, sums);
             if (rec && valid) {
                 const int64_t e = 25 * (int64_t)step + (k - 1);
                 rec[(e * nrec_sys + blockIdx.x) * kSmallMax + t] = tmn[t];
@@ -2149,9 +2150,18 @@ void launch_stage(const View &d, int k, double dt, double dt_k, double tadd, con
 void launch_step_count(int64_t *istep, hipStream_t st) {
     hipLaunchKernelGGL(k_step_count, dim3(1), dim3(64), 0, st, istep);
 }
-void launch_small(const View &d, const int *sys_ids, int nsys, int n_steps, double dt, const DtK &dtk,
+// 128 threads for systems of at most 16 bodies (their pairs fit one round),
+// 256 above (fewer rounds of pairs; the extra waves cost the small ones time
+// at every barrier)
+void launch_small(const View &d, const int *sys_ids, int nsys, int max_n, int n_steps, double dt, const DtK &dtk,
                   int64_t i_step0, JMain *rec, hipStream_t st) {
-    if (nsys) hipLaunchKernelGGL(k_small, dim3(nsys), dim3(2 * kSmallMax), 0, st, d, sys_ids, n_steps, dt, dtk, i_step0, rec, nsys);
+    if (!nsys) return;
+    if (max_n <= 16)
+        hipLaunchKernelGGL(k_small<2 * kSmallMax>, dim3(nsys), dim3(2 * kSmallMax), 0, st, d, sys_ids, n_steps, dt, dtk,
+                           i_step0, rec, nsys);
+    else
+        hipLaunchKernelGGL(k_small<4 * kSmallMax>, dim3(nsys), dim3(4 * kSmallMax), 0, st, d, sys_ids, n_steps, dt, dtk,
+                           i_step0, rec, nsys);
 }
 void launch_extrema(const View &d, int invert, hipStream_t st) {
     hipLaunchKernelGGL(k_extrema, dim3((d.nb + 255) / 256), dim3(256), 0, st, d, invert);

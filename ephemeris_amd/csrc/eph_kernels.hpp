@@ -98,7 +98,7 @@ struct PBlock { int64_t rec_entry0; int32_t n_steps, n_valid; };
 // n_steps < 0 computes the first acceleration from record entry 0
 void launch_particles(const PView &pv, int n_steps, double dt, const DtK &dtk, hipStream_t st,
                       const PBlock *blocks = nullptr);
-void launch_small(const View &d, const int *sys_ids, int nsys, int n_steps, double dt, const DtK &dtk,
+void launch_small(const View &d, const int *sys_ids, int nsys, int max_n, int n_steps, double dt, const DtK &dtk,
                   int64_t i_step0, JMain *rec, hipStream_t st);
 
 }  // namespace eph

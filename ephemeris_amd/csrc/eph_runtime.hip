@@ -431,7 +431,7 @@ int Batch::integrate_small(double dt, int64_t n_step, JMain *rec, int64_t rec_st
         const int64_t ns = std::min(chunk, n_step - s0);
         {
             Scope sc(profiler, 5, stream);
-            launch_small(v, d_small_ids, (int)small_ids.size(), (int)ns, dt, dtk, s0, rec, stream);
+            launch_small(v, d_small_ids, (int)small_ids.size(), max_sys_n, (int)ns, dt, dtk, s0, rec, stream);
         }
         if (after_chunk) after_chunk(ctx, s0, ns);
     }
@@ -971,7 +971,7 @@ int integrate_particles(Particles *ps, double dt, int64_t n_step) {
         if (c >= 2) HIP_CHECK(hipStreamWaitEvent(f->stream, ps->ev_free[b], 0));
         {
             Scope sc(f->profiler, 5, f->stream);
-            launch_small(v, f->d_small_ids, (int)f->small_ids.size(), (int)ns, dt, dtk, s0, rec, f->stream);
+            launch_small(v, f->d_small_ids, (int)f->small_ids.size(), f->max_sys_n, (int)ns, dt, dtk, s0, rec, f->stream);
         }
         HIP_CHECK(hipEventRecord(ps->ev_rec[b], f->stream));
         HIP_CHECK(hipStreamWaitEvent(ps->pstream, ps->ev_rec[b], 0));
