@@ -12,8 +12,10 @@
   deform kernel's cached pair geometry, so its non-point pass runs in its
   own kernels, its constant-time-lag bodies re-form the pair geometry on
   every spin pass, and its main force takes four rounds of 256 partners.
-* A batch of all of them together: each member ends with the bits it has
-  alone.
+* 40 bodies with geopotentials, the ring and constant-time-lag tides, in the
+  single-thread order on the one-workgroup-per-system path at 256 threads.
+* A batch of the 1-, 2-, 3- and 804-body systems: each member ends with the
+  bits it has alone.
 * Empty inputs are rejected.
 
 Force evaluations must equal the oracle's bit for bit (ring-carrying
@@ -87,6 +89,21 @@ def test_systems_of_one_two_and_three_bodies():
     for idx in ([0], [0, host], [0, host, moon]):
         print(f"\n[case] {len(idx)}-body system {idx} of full504", flush=True)
         _check_against_oracle(full.subset(idx), f"{len(idx)} bodies", steps=4)
+
+
+def test_forty_bodies_with_models_and_tides():
+    """40 bodies of the 504-body system -- the star (geopotential), the
+    planets (geopotentials, the ring) and the first moons (constant-time-lag
+    tides): below 49 bodies the reference sums in the single-thread order, and
+    the library integrates it on the one-workgroup-per-system path with 256
+    threads.  Accel and four steps against the oracle."""
+    print("\n[case] 40-body subset of full504 with models and tides, single-thread order", flush=True)
+    full = po.OSystem.load(CONFIGS["full504"], pool_at_load=False)
+    ms = full.masses
+    idx = list(range(40))
+    sub = ms[idx]
+    assert (sub["gp"] >= 0).sum() >= 2 and (sub["ring"] >= 0).any() and (sub["tide_delay"] != 0).sum() >= 2
+    _check_against_oracle(full.subset(idx), "40 bodies", steps=4)
 
 
 def test_system_above_the_fused_and_cached_limits(tmp_path):
