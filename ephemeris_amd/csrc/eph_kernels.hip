@@ -1704,6 +1704,155 @@ __device__ void deform_body_tile(const Acc &A, int gi, int il, int n, const JDef
     deform_write(A, gi, 1, vi, phi, nacc, C, GI, w);
 }
 
+// Deformation of a small system by the whole workgroup, pass by pass: in
+// each pass the threads form the terms of the pairs (i, q) of the bodies
+// still iterating -- -phi, naccel and the subtracted tidal matrix of
+// deform_by (mass.cpp:78-103), the last one at the body's current spin for a
+// constant-time-lag body -- into LDS, thirteen threads per body add them in
+// partner order, and the body's own thread runs UPDATE_HARMONICS and
+// UPDATE_ANGULAR_VELOCITY.  A body without delayed tides sums once and
+// iterates its spin on those sums (its terms do not depend on the spin);
+// a constant-time-lag body re-sums its tidal matrix at every pass, as the
+// reference does, up to MAX_ANGULAR_VELOCITY_ITER passes (phi and naccel do
+// not change between passes).
+struct SmallDefShared {
+    double w[kSmallMax * 3], k2[kSmallMax], td[kSmallMax];
+    int act[kSmallMax], list[kSmallMax], na;
+};
+
+__device__ __forceinline__ void small_deform_terms(const JDeform &Ti, const JDeform &T, double k2, double td, v3 w,
+                                                   bool first, double *o) {
+    const ddv3 ri{dd{Ti.rxh, Ti.rxl}, dd{Ti.ryh, Ti.ryl}, dd{Ti.rzh, Ti.rzl}};
+    const ddv3 rj{dd{T.rxh, T.rxl}, dd{T.ryh, T.ryl}, dd{T.rzh, T.rzl}};
+    const v3 r = diff(rj, ri);
+    const double rr2 = 1 / smax(dot(r, r), Ti.sR2 + T.sR2);
+    const double rr = sqrt(rr2);
+    const double tp_dphi = rr * T.GM;
+    const double tp_dg = rr2 * tp_dphi;
+    if (first) {
+        const v3 na = tp_dg * r;
+        o[0] = -tp_dphi;                                   // phi -= tp_dphi
+        o[1] = na.x; o[2] = na.y; o[3] = na.z;             // naccel += tp_dg r
+    }
+    m3 X = mzero();                                        // Csum -= X
+    if (k2 != 0 && td != 0) {                              // DAMPED_TIDAL_DEFORMATION_MATRIX
+        const ddv3 vi{dd{Ti.vxh, Ti.vxl}, dd{Ti.vyh, Ti.vyl}, dd{Ti.vzh, Ti.vzl}};
+        const ddv3 vj{dd{T.vxh, T.vxl}, dd{T.vyh, T.vyl}, dd{T.vzh, T.vzl}};
+        const v3 v = diff(vj, vi);
+        const double fmj = -3 * tp_dg;
+        v3 dw = rr2 * cross(r, cross(w, r) - v);
+        double dwt = normalize(dw);
+        dwt *= td * T.tdf;
+        const double ecwt2 = 1 + 4 * dwt * dwt,
+                     secwt2 = sqrt(ecwt2),
+                     x0 = sqrt((1 + secwt2) / (2 * ecwt2)),
+                     y0 = dwt * sqrt(2 / (ecwt2 * (1 + secwt2))),
+                     z02 = dwt * dwt * (2 / (ecwt2 + secwt2));
+        const v3 dr = x0 * r + y0 * cross(dw, r);
+        m3 t = outer(rr2 * dr, dr) - outer(z02 * dw, dw);
+        t = adddiag(t, z02 - 1.0 / 3);
+        X = (fmj * k2) * t;
+    } else if (k2 != 0) {                                  // _NDELAY
+        const double fmj = -3 * tp_dg;
+        X = (fmj * k2) * subdiag(outer(rr2 * r, r), 1.0 / 3);
+    }
+    const double *x = &X.x.x;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) o[4 + c] = -x[c];
+}
+
+// called by every thread of the workgroup of NT threads (it synchronises)
+template <int NT>
+__device__ void deform_small(const Acc &A, int off, int n, const JDeform *tile, double *terms, double *sums,
+                             SmallDefShared &sh) {
+    const int tid = (int)threadIdx.x;
+    const bool body = tid < n;
+    const int gi = off + (body ? tid : 0);
+    // the body's own state, held by its thread across the passes
+    ddv3 vi{}, GL{};
+    Harmonics H{};
+    v3 w{0, 0, 0}, nacc{0, 0, 0};
+    double phi = 0, k2 = 0, td = 0;
+    m3 C = mzero(), GI = mzero();
+    if (body) {
+        vi = A.lddv(F_V, gi);
+        k2 = A.ld(F_K2, gi);
+        td = A.ld(F_TD, gi);
+        GL = A.lddv(F_GL, gi);
+        H = harmonics_load(A, gi);
+        w = A.ldv(F_W, gi);
+        sh.w[3 * tid] = w.x; sh.w[3 * tid + 1] = w.y; sh.w[3 * tid + 2] = w.z;
+        sh.k2[tid] = k2;
+        sh.td[tid] = td;
+        sh.act[tid] = 1;
+        sh.list[tid] = tid;
+    }
+    if (tid == 0) sh.na = n;
+    __syncthreads();
+    const int m = n - 1;                                   // pairs per body
+    const int bpr = m > 0 ? min(NT / m, NT / 13) : 1;      // bodies per round
+    for (int pass = 0; pass < 4; ++pass) {
+        const int na = sh.na;
+        if (na == 0) break;
+        const bool first = pass == 0;
+        for (int b0 = 0; b0 < na && m > 0; b0 += bpr) {
+            const int nb = min(bpr, na - b0);
+            if (tid < nb * m) {
+                const int il = sh.list[b0 + tid / m], qq = tid - (tid / m) * m;
+                const int q = qq < il ? qq : qq + 1;
+                small_deform_terms(tile[il], tile[q], sh.k2[il], sh.td[il],
+                                   v3{sh.w[3 * il], sh.w[3 * il + 1], sh.w[3 * il + 2]}, first, terms + tid * 13);
+            }
+            __syncthreads();
+            if (tid < 13 * nb) {
+                const int j = tid / 13, c = tid - j * 13;
+                const int il = sh.list[b0 + j];
+                if (first || c >= 4) {
+                    const double *P = terms + j * m * 13 + c;
+                    double acc = 0;
+                    for (int qq = 0; qq < m; ++qq) acc += P[qq * 13];
+                    sums[il * 13 + c] = acc;
+                }
+            }
+            __syncthreads();
+        }
+        if (m == 0 && body) {                              // no partner: the sums stay +0
+            for (int c = 0; c < 13; ++c) sums[tid * 13 + c] = 0.0;
+        }
+        __syncthreads();
+        if (body && sh.act[tid]) {
+            const double *S = sums + tid * 13;
+            if (first) {
+                phi = S[0];
+                nacc = v3{S[1], S[2], S[3]};
+            }
+            const m3 Csum{v3{S[4], S[5], S[6]}, v3{S[7], S[8], S[9]}, v3{S[10], S[11], S[12]}};
+            if (!(k2 != 0 && td != 0)) {                   // the spin iterates on the same sums
+                int iters = 4;                             // MAX_ANGULAR_VELOCITY_ITER
+                for (;;) {
+                    H.apply(Csum, w, C, GI);
+                    if (update_spin(A, gi, GI, w, GL) || --iters == 0) break;
+                }
+                sh.act[tid] = 0;
+            } else {
+                H.apply(Csum, w, C, GI);
+                const bool brk = update_spin(A, gi, GI, w, GL);
+                sh.act[tid] = (brk || pass == 3) ? 0 : 1;
+                sh.w[3 * tid] = w.x; sh.w[3 * tid + 1] = w.y; sh.w[3 * tid + 2] = w.z;
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int k = 0;
+            for (int i = 0; i < n; ++i)
+                if (sh.act[i]) sh.list[k++] = i;
+            sh.na = k;
+        }
+        __syncthreads();
+    }
+    if (body) deform_write(A, gi, 1, vi, phi, nacc, C, GI, w);
+}
+
 // Main force of a small system in the single-thread order, by the whole
 // workgroup: each thread forms every term of one ordered pair (i, q) -- the
 // own-loop terms of i's partner q and the reaction terms of outer body q on
@@ -1838,7 +1987,8 @@ __global__ __launch_bounds__(NT) void k_small(View d, const int *__restrict__ sy
     __shared__ JDeform tdf[kSmallMax];
     __shared__ JMain tmn[kSmallMax];
     __shared__ double terms[NT * kSmallTerm];
-    __shared__ double sums[kSmallMax * 11];
+    __shared__ double sums[kSmallMax * 13];
+    __shared__ SmallDefShared dsh;
     for (int step = 0; step < n_steps; ++step) {
         for (int k = 1; k <= 25; ++k) {
             const double tadd = dt * (dtk.v[k] + (double)(i_step0 + step));
@@ -1847,7 +1997,7 @@ __global__ __launch_bounds__(NT) void k_small(View d, const int *__restrict__ sy
             __syncthreads();
             if (valid) load_jdeform(A, tdf[t], off + t);
             __syncthreads();
-            if (valid) deform_body_tile(A, off + t, t, n, tdf);
+            deform_small<NT>(A, off, n, tdf, terms, sums, dsh);
             __syncthreads();
             if (valid) load_jmain(A, tmn[t], off + t);
             __syncthreads();
