@@ -1632,78 +1632,6 @@ __global__ void k_step_count(int64_t *istep) {
 // whole-system-in-one-workgroup path: every body of the system sits in one
 // LDS tile, so deform and main force need no further tile traffic
 // ---------------------------------------------------------------------------
-__device__ void deform_body_tile(const Acc &A, int gi, int il, int n, const JDeform *tile) {
-    const ddv3 ri = A.lddv(F_R, gi), vi = A.lddv(F_V, gi);
-    const double sR2 = A.ld(F_SR2, gi), k2 = A.ld(F_K2, gi), td = A.ld(F_TD, gi);
-    const ddv3 GL = A.lddv(F_GL, gi);
-    Harmonics H;
-    H = harmonics_load(A, gi);
-    v3 w = A.ldv(F_W, gi);
-    double phi = 0;
-    v3 nacc{0, 0, 0};
-    m3 C, GI, Csum;
-    if (!(k2 != 0 && td != 0)) {                           // spin-independent partner sum
-        Csum = mzero();
-        for (int q = 0; q < n; ++q) {
-            if (q == il) continue;
-            const JDeform &T = tile[q];
-            const ddv3 rj{dd{T.rxh, T.rxl}, dd{T.ryh, T.ryl}, dd{T.rzh, T.rzl}};
-            const v3 r = diff(rj, ri);
-            const double rr2 = 1 / smax(dot(r, r), sR2 + T.sR2);
-            const double rr = sqrt(rr2);
-            const double tp_dphi = rr * T.GM;
-            const double tp_dg = rr2 * tp_dphi;
-            phi -= tp_dphi;
-            nacc += tp_dg * r;
-            if (k2 != 0) {
-                const double fmj = -3 * tp_dg;
-                Csum = Csum - (fmj * k2) * subdiag(outer(rr2 * r, r), 1.0 / 3);
-            }
-        }
-        int iters = 4;
-        for (;;) {
-            H.apply(Csum, w, C, GI);
-            if (update_spin(A, gi, GI, w, GL) || --iters == 0) break;
-        }
-    } else {
-        for (int iters = 4; iters > 0; --iters) {
-            phi = 0;
-            nacc = v3{0, 0, 0};
-            Csum = mzero();
-            for (int q = 0; q < n; ++q) {
-                if (q == il) continue;
-                const JDeform &T = tile[q];
-                const ddv3 rj{dd{T.rxh, T.rxl}, dd{T.ryh, T.ryl}, dd{T.rzh, T.rzl}};
-                const ddv3 vj{dd{T.vxh, T.vxl}, dd{T.vyh, T.vyl}, dd{T.vzh, T.vzl}};
-                const v3 r = diff(rj, ri);
-                const v3 v = diff(vj, vi);
-                const double rr2 = 1 / smax(dot(r, r), sR2 + T.sR2);
-                const double rr = sqrt(rr2);
-                const double tp_dphi = rr * T.GM;
-                phi -= tp_dphi;
-                const double tp_dg = rr2 * tp_dphi;
-                nacc += tp_dg * r;
-                const double fmj = -3 * tp_dg;
-                v3 dw = rr2 * cross(r, cross(w, r) - v);
-                double dwt = normalize(dw);
-                dwt *= td * T.tdf;
-                const double ecwt2 = 1 + 4 * dwt * dwt,
-                             secwt2 = sqrt(ecwt2),
-                             x0 = sqrt((1 + secwt2) / (2 * ecwt2)),
-                             y0 = dwt * sqrt(2 / (ecwt2 * (1 + secwt2))),
-                             z02 = dwt * dwt * (2 / (ecwt2 + secwt2));
-                const v3 dr = x0 * r + y0 * cross(dw, r);
-                m3 t = outer(rr2 * dr, dr) - outer(z02 * dw, dw);
-                t = adddiag(t, z02 - 1.0 / 3);
-                Csum = Csum - (fmj * k2) * t;
-            }
-            H.apply(Csum, w, C, GI);
-            if (update_spin(A, gi, GI, w, GL)) break;
-        }
-    }
-    deform_write(A, gi, 1, vi, phi, nacc, C, GI, w);
-}
-
 // Deformation of a small system by the whole workgroup, pass by pass: in
 // each pass the threads form the terms of the pairs (i, q) of the bodies
 // still iterating -- -phi, naccel and the subtracted tidal matrix of
