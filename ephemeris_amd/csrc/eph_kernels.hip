@@ -291,12 +291,13 @@ constexpr int kDefK4Pitch = 13;
 constexpr int kJDefF = 8;
 __constant__ int c_jdef_field[kJDefF] = {F_R + 0, F_R + 1, F_R + 2, F_R + 3, F_R + 4, F_R + 5, F_SR2, F_GM};
 
-// Three wavefronts per SIMD (168 registers; the few values the compiler then
-// keeps in scratch are stored before the partner loop and reloaded after it).
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_deform_k4(View d, const int4 *__restrict__ blk, int prepare, NpTail np) {
-    if (np_tail(d, np)) return;
+// Block `bid` of the list; term holds 64 x kDefK4Pitch doubles.  Its launch
+// runs three wavefronts per SIMD (168 registers; the few values the compiler
+// then keeps in scratch are stored before the partner loop and reloaded
+// after it).
+__device__ __forceinline__ void deform_k4_body(View d, const int4 *__restrict__ blk, int prepare, int bid, double *term) {
     const Acc A{d.F, d.nbp};
-    const int4 bk = blk[blockIdx.x];
+    const int4 bk = blk[bid];
     if (bk.z <= 0) return;                                 // padding of an XCD-dealt list
     const SysMeta sm = d.sys[bk.x];
     const int n = sm.n, off = sm.off;
@@ -305,7 +306,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     const int gi = valid ? d.perm[off + bk.y + b] : off;
     const int il = gi - off;
     using Blk = PartnerBlock<4, kJDefF>;
-    __shared__ double term[64 * kDefK4Pitch];
     __shared__ double jb[4 * Blk::P];
     Blk pb;
     pb.init(c_jdef_field, d.nbp, lane);
@@ -1099,16 +1099,15 @@ constexpr int kWpbDefPitch = 65;
 // TAIL: the launch carries the non-point item fields in extra workgroups
 // (without them the kernel needs no scratch).  Work items with il < 0 are
 // padding of an XCD-dealt list.
-template <int W, bool TAIL>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 1 ? 3 : 1))) void k_deform_wpb(View d, const int2 *__restrict__ work, int prepare, NpTail np) {
-    if (TAIL && np_tail(d, np)) return;
+template <int W>
+__device__ __forceinline__ void deform_wpb_body(View d, const int2 *__restrict__ work, int prepare, int bid,
+                                                double (*term)[13 * kWpbDefPitch]) {
     const Acc A{d.F, d.nbp};
-    const int s = work[blockIdx.x].x, il = work[blockIdx.x].y;
+    const int s = work[bid].x, il = work[bid].y;
     if (il < 0) return;
     const SysMeta sm = d.sys[s];
     const int n = sm.n, off = sm.off, gi = off + il;
     const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
-    __shared__ double term[W][13 * kWpbDefPitch];
     __shared__ double sums[13];
     __shared__ int again_s;
     __shared__ double wsh[3];
@@ -1268,6 +1267,30 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 1 ?
         w = v3{wsh[0], wsh[1], wsh[2]};
         __syncthreads();
     }
+}
+
+template <int W, bool TAIL>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 1 ? 3 : 1))) void k_deform_wpb(View d, const int2 *__restrict__ work, int prepare, NpTail np) {
+    if (TAIL && np_tail(d, np)) return;
+    __shared__ double term[W][13 * kWpbDefPitch];
+    deform_wpb_body<W>(d, work, prepare, (int)blockIdx.x, term);
+}
+
+// The large-batch deformation in one launch: workgroups [0, nwork) take the
+// constant-time-lag bodies of `work` (one wavefront each, the longest
+// workgroups, dispatched first), [nwork, np.ndef) the four-lane blocks of
+// the other bodies, which fill the CUs while the last spin passes run, and
+// the rest the item fields.  nwork is a multiple of 8, so the block list
+// keeps the XCD placement it was dealt with.  The two kinds share one term
+// buffer.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_deform_mixed(
+    View d, const int2 *__restrict__ work, int nwork, const int4 *__restrict__ blk, int prepare, NpTail np) {
+    if (np_tail(d, np)) return;
+    constexpr int kTerm = 13 * kWpbDefPitch > 64 * kDefK4Pitch ? 13 * kWpbDefPitch : 64 * kDefK4Pitch;
+    __shared__ double term[kTerm];
+    const int b = (int)blockIdx.x;
+    if (b < nwork) deform_wpb_body<1>(d, work, prepare, b, reinterpret_cast<double (*)[13 * kWpbDefPitch]>(term));
+    else deform_k4_body(d, blk, prepare, b - nwork, term);
 }
 
 // ---------------------------------------------------------------------------
@@ -2154,10 +2177,11 @@ void set_rk_table(const double *tab650, hipStream_t st) {
     (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(c_row0), row0, sizeof(row0), 0, hipMemcpyHostToDevice, st);
 }
 
-void launch_deform_k4(const View &d, const int4 *blk16, int nblk, int prepare, int np_rows, int max_n, hipStream_t st) {
-    const NpTail np{nblk, np_rows, (max_n + 63) / 64};
-    const int grid = nblk + np_rows * np.chunks;
-    if (grid) hipLaunchKernelGGL(k_deform_k4, dim3(grid), dim3(64), 0, st, d, blk16, prepare, np);
+void launch_deform_mixed(const View &d, const int2 *work, int nwork, const int4 *blk16, int nblk, int prepare,
+                         int np_rows, int max_n, hipStream_t st) {
+    const NpTail np{nwork + nblk, np_rows, (max_n + 63) / 64};
+    const int grid = nwork + nblk + np_rows * np.chunks;
+    if (grid) hipLaunchKernelGGL(k_deform_mixed, dim3(grid), dim3(64), 0, st, d, work, nwork, blk16, prepare, np);
 }
 void launch_deform(const View &d, const int4 *blk_light, int n_light, const int4 *blk_damped, int n_damped,
                    int prepare, hipStream_t st) {

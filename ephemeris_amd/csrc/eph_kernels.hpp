@@ -61,9 +61,12 @@ void set_rk_table(const double *tab650, hipStream_t st);
 void launch_deform(const View &d, const int4 *blk_light, int n_light, const int4 *blk_damped, int n_damped,
                    int prepare, hipStream_t st);
 void launch_main_pool(const View &d, const int2 *blk, int nblk, hipStream_t st);
-// deform of bodies without constant-time lag, four lanes per body;
-// blk16 = (system, first slot, count <= 16, 0) in the class-sorted slots
-void launch_deform_k4(const View &d, const int4 *blk16, int nblk, int prepare, int np_rows, int max_n, hipStream_t st);
+// constant-time-lag bodies (work = {system, body}, nwork a multiple of 8),
+// one wavefront each, and the other bodies four lanes each (blk16 =
+// (system, first slot, count <= 16, 0) in the class-sorted slots) in one
+// launch, with the item fields
+void launch_deform_mixed(const View &d, const int2 *work, int nwork, const int4 *blk16, int nblk, int prepare,
+                         int np_rows, int max_n, hipStream_t st);
 // four lanes per body; blk16 = (system, first body) of every 16 bodies
 void launch_main_k4(const View &d, const int2 *blk16, int nblk, hipStream_t st);
 void launch_main_serial(const View &d, const int2 *blk, int nblk, hipStream_t st);

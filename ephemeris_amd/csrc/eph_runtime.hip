@@ -360,8 +360,8 @@ int Batch::accel(int mode, bool prepare_only) {
             // constant-time-lag bodies re-sum their partners once per spin pass:
             // one wavefront each; the rest four lanes each (that launch
             // carries the item fields)
-            launch_deform_wpb(v, d_work_damped, n_work_damped, max_sys_n, prepare_only ? 0 : 1, 0, 0, stream);
-            launch_deform_k4(v, d_blk_light16, n_blk_light16, prepare_only ? 0 : 1, np_rows, pl.max_n, stream);
+            launch_deform_mixed(v, d_work_damped, n_work_damped, d_blk_light16, n_blk_light16,
+                                prepare_only ? 0 : 1, np_rows, pl.max_n, stream);
             field_done = true;
         }
     }
@@ -636,6 +636,9 @@ int Batch::build_class_tables(const eph_mass *masses) {
                            [&](int2 b) { return (double)sys_n[b.x]; }, make_int2(0, -1));
     light16 = xcd_deal(light16, n_sys, [](int4 b) { return b.x; },
                        [&](int4 b) { return (double)b.z * sys_n[b.x]; }, make_int4(0, 0, 0, 0));
+    // the merged deform launch puts the block list after this one: a length
+    // that is a multiple of 8 keeps each block on the XCD it was dealt to
+    while (damped_work.size() % kXcds) damped_work.push_back(make_int2(0, -1));
     HIP_CHECK(hipStreamSynchronize(stream));
     for (void *p : class_allocs) (void)hipFree(p);
     class_allocs.clear();
