@@ -159,12 +159,16 @@ def batched_leg(steps, warmup, n_clones):
             "rk12_member_steps_per_s": steps * ens.n_members / el, "ms_per_step": el / steps * 1e3,
             "load_s": t_load, "kernel_variant": {2: "one wavefront per body", 4: "four lanes per body"}.get(variant, variant),
             "roofline_main_force": {"achieved": achieved, "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
-                                    "frac": achieved / FP64_VECTOR_PEAK_TFLOPS, "avg_launch_ms": main_ms},
+                                    "frac": achieved / FP64_VECTOR_PEAK_TFLOPS, "avg_launch_ms": main_ms,
+                                    "note": ("main-force FLOP over the whole launch, which also forms the non-point "
+                                             "items' partial sums in extra workgroups (about 2% of its time)"
+                                             if variant == 4 else "main-force FLOP over the main-force launch")},
             "roofline_deform": {"pairs_by_class": cls_pairs, "flop_per_pair": FLOP_DEFORM_PAIR,
                                 "flop_first_pass": deform_flop, "ms_per_evaluation": deform_ms,
                                 "achieved_lower_bound": deform_tf, "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
                                 "frac_lower_bound": deform_tf / FP64_VECTOR_PEAK_TFLOPS,
-                                "note": "both deform launches; constant-time-lag bodies counted for one spin pass"},
+                                "note": "the deformation launch (constant-time-lag bodies, the other bodies and the "
+                                        "item fields); constant-time-lag bodies counted for one spin pass"},
             "kernel_ms": {k: round(v[0], 5) for k, v in prof.items()}}
 
 

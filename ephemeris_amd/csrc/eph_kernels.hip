@@ -588,9 +588,10 @@ __device__ __forceinline__ PairTerms pair_terms_k4(const IMain &I, bool own_quad
     return o;
 }
 
-__global__ __launch_bounds__(64) void k_main_k4(View d, const int2 *__restrict__ blk) {
+// block `bid` of the list; term holds 64 x kK4Pitch doubles
+__device__ __forceinline__ void main_k4_body(View d, const int2 *__restrict__ blk, int bid, double *term) {
     const Acc A{d.F, d.nbp};
-    const int s = blk[blockIdx.x].x, i0 = blk[blockIdx.x].y;   // bodies i0 .. i0 + 15
+    const int s = blk[bid].x, i0 = blk[bid].y;             // bodies i0 .. i0 + 15
     if (i0 < 0) return;                                    // padding of an XCD-dealt list
     const SysMeta sm = d.sys[s];
     const int n = sm.n, off = sm.off;
@@ -598,7 +599,6 @@ __global__ __launch_bounds__(64) void k_main_k4(View d, const int2 *__restrict__
     const int il = i0 + b;
     const bool valid = il < n;
     const int gi = off + (valid ? il : 0);
-    __shared__ double term[64 * kK4Pitch];
     const IMain I = load_imain(A, gi);
     bool own_quad = false;
     {
@@ -666,6 +666,11 @@ __global__ __launch_bounds__(64) void k_main_k4(View d, const int2 *__restrict__
             A.stv(F_NACC, gi, A.ldv(F_NACCC, gi) * kC2);
         }
     }
+}
+
+__global__ __launch_bounds__(64) void k_main_k4(View d, const int2 *__restrict__ blk) {
+    __shared__ double term[64 * kK4Pitch];
+    main_k4_body(d, blk, (int)blockIdx.x, term);
 }
 
 // ---------------------------------------------------------------------------
@@ -1412,15 +1417,15 @@ __global__ __launch_bounds__(64) void k_np_field(View d, int nrows) {
     np_field_one(d, A, row, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-__global__ __launch_bounds__(64) void k_np_item(View d, int nrows, int maxT) {
-    // one wavefront per (item, round k): the item's partial sums over the
-    // slice it visits in round k (tpmi of accel_nonpoint).  Lanes form the
-    // partners' terms (subtracted terms stored negated, so a - b becomes
-    // a + (-b), the same value); lane c < 6 adds chain c in partner order
-    // (td x, y, z, tq x, y, z).  The self pair and lanes past the slice add
-    // +0, an exact identity for sums that start at +0.
+// one wavefront per (item, round k): the item's partial sums over the slice
+// it visits in round k (tpmi of accel_nonpoint).  Lanes form the partners'
+// terms (subtracted terms stored negated, so a - b becomes a + (-b), the same
+// value); lane c < 6 adds chain c in partner order (td x, y, z, tq x, y, z).
+// The self pair and lanes past the slice add +0, an exact identity for sums
+// that start at +0.  bid = item row * maxT + round; term holds 64 x 7 doubles.
+__device__ __forceinline__ void np_item_body(View d, int nrows, int maxT, int bid, double *term) {
     const Acc A{d.F, d.nbp};
-    const int k = (int)blockIdx.x % maxT, it_i = (int)blockIdx.x / maxT;
+    const int k = bid % maxT, it_i = bid / maxT;
     if (it_i >= nrows) return;
     const NPItem it = d.items[it_i];
     const SysMeta sm = d.sys[it.sys];
@@ -1431,7 +1436,6 @@ __global__ __launch_bounds__(64) void k_np_item(View d, int nrows, int maxT) {
     const int j0 = kk * sm.n / T, j1 = (kk + 1) * sm.n / T;
     const int lane = (int)threadIdx.x;
     const ddv3 rp = A.lddv(F_R, it.body);
-    __shared__ double term[64 * 7];
     double acc = 0;
     for (int base = j0; base < j1; base += 64) {
         const int jl = base + lane;
@@ -1456,6 +1460,25 @@ __global__ __launch_bounds__(64) void k_np_item(View d, int nrows, int maxT) {
         __syncthreads();
     }
     if (lane < 6) d.part[6 * ((int64_t)it_i * maxT + k) + lane] = acc;
+}
+
+__global__ __launch_bounds__(64) void k_np_item(View d, int nrows, int maxT) {
+    __shared__ double term[64 * 7];
+    np_item_body(d, nrows, maxT, (int)blockIdx.x, term);
+}
+
+// The large-batch main force with the items' partial sums in extra
+// workgroups: [0, nblk) the four-lane blocks (in the XCD placement they were
+// dealt with), then one wavefront per (item row, round).  The item sums read
+// only positions, masses and the item fields formed in the deformation
+// launch, so they fill the CUs while the last blocks run.  One term buffer
+// serves both.
+static_assert(kK4Pitch >= 7, "the item sums use 7 doubles per lane of the term buffer");
+__global__ __launch_bounds__(64) void k_main_k4_items(View d, const int2 *__restrict__ blk, int nblk, int nrows, int maxT) {
+    __shared__ double term[64 * kK4Pitch];
+    const int b = (int)blockIdx.x;
+    if (b < nblk) main_k4_body(d, blk, b, term);
+    else np_item_body(d, nrows, maxT, b - nblk, term);
 }
 
 constexpr int kRecvStaged = 64;          // items per system staged in LDS
@@ -2191,6 +2214,10 @@ void launch_deform(const View &d, const int4 *blk_light, int n_light, const int4
 void launch_main_pool(const View &d, const int2 *blk, int nblk, hipStream_t st) {
     if (nblk) hipLaunchKernelGGL(k_main_pool<kBS>, dim3(nblk), dim3(kBS), 0, st, d, blk);
 }
+void launch_main_k4_items(const View &d, const int2 *blk16, int nblk, int nrows, int maxT, hipStream_t st) {
+    const int grid = nblk + nrows * maxT;
+    if (grid) hipLaunchKernelGGL(k_main_k4_items, dim3(grid), dim3(64), 0, st, d, blk16, nblk, nrows, maxT);
+}
 void launch_main_k4(const View &d, const int2 *blk16, int nblk, hipStream_t st) {
     if (nblk) hipLaunchKernelGGL(k_main_k4, dim3(nblk), dim3(64), 0, st, d, blk16);
 }
@@ -2240,10 +2267,10 @@ void launch_np_field(const View &d, int nrows, int max_n, hipStream_t st) {
     if (nrows) hipLaunchKernelGGL(k_np_field, dim3((max_n + 63) / 64, nrows), dim3(64), 0, st, d, nrows);
 }
 void launch_np(const View &d, int nrows, int max_n, int maxT, const int2 *recv_blk, int nrecv, bool with_field,
-               hipStream_t st) {
+               hipStream_t st, bool with_item) {
     if (!nrows) return;
     if (with_field) hipLaunchKernelGGL(k_np_field, dim3((max_n + 63) / 64, nrows), dim3(64), 0, st, d, nrows);
-    hipLaunchKernelGGL(k_np_item, dim3(nrows * maxT), dim3(64), 0, st, d, nrows, maxT);
+    if (with_item) hipLaunchKernelGGL(k_np_item, dim3(nrows * maxT), dim3(64), 0, st, d, nrows, maxT);
     if (nrecv) hipLaunchKernelGGL(k_np_recv, dim3(nrecv), dim3(kBS), 0, st, d, recv_blk, maxT);
 }
 void launch_stage(const View &d, int k, double dt, double dt_k, double tadd, const int64_t *istep, hipStream_t st) {

@@ -69,6 +69,9 @@ void launch_deform_mixed(const View &d, const int2 *work, int nwork, const int4 
                          int np_rows, int max_n, hipStream_t st);
 // four lanes per body; blk16 = (system, first body) of every 16 bodies
 void launch_main_k4(const View &d, const int2 *blk16, int nblk, hipStream_t st);
+// the same with the items' partial sums of launch_np (rows, rounds) in extra
+// workgroups of the launch
+void launch_main_k4_items(const View &d, const int2 *blk16, int nblk, int nrows, int maxT, hipStream_t st);
 void launch_main_serial(const View &d, const int2 *blk, int nblk, hipStream_t st);
 // latency path, one workgroup of 1-4 wavefronts per body; work = {system,
 // body index in system}
@@ -88,8 +91,10 @@ constexpr int kNpFusedMaxN_h = 512, kNpFusedMaxItems_h = 64;
 // (waves as for launch_main_wpb)
 void launch_deform_wpb(const View &d, const int2 *work, int nwork, int max_n, int prepare, int np_rows, int waves,
                        hipStream_t st);
+// with_item = false: the items' partial sums were formed already
+// (launch_main_k4_items)
 void launch_np(const View &d, int nrows, int max_n, int maxT, const int2 *recv_blk, int nrecv, bool with_field,
-               hipStream_t st);
+               hipStream_t st, bool with_item = true);
 // istep: optional device step counter (graph replays); launch_step_count adds 1
 void launch_stage(const View &d, int k, double dt, double dt_k, double tadd, const int64_t *istep, hipStream_t st);
 void launch_step_count(int64_t *istep, hipStream_t st);

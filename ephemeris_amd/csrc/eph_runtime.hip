@@ -367,6 +367,7 @@ int Batch::accel(int mode, bool prepare_only) {
     }
     if (prepare_only) return EPH_OK;
     const bool fused_np = wpb && pl.np_fusable && variant != 3;
+    bool items_done = false;
     if (fused_np && pl.nrows && !field_done) {
         Scope sc(profiler, 4, stream);
         launch_np_field(v, pl.nrows, pl.max_n, stream);
@@ -375,7 +376,10 @@ int Batch::accel(int mode, bool prepare_only) {
         Scope sc(profiler, 2, stream);
         if (wpb) launch_main_wpb(v, pl.d_work_pool, pl.n_work_pool, pl.max_n, fused_np, eff_waves(), stream);
         else if (variant == 1) launch_main_pool(v, pl.d_blk_pool, pl.n_pool, stream);
-        else launch_main_k4(v, pl.d_blk16, pl.n_blk16, stream);
+        else if (pl.nrows && !fused_np) {                      // the items' sums ride along
+            launch_main_k4_items(v, pl.d_blk16, pl.n_blk16, pl.nrows, pl.maxT, stream);
+            items_done = true;
+        } else launch_main_k4(v, pl.d_blk16, pl.n_blk16, stream);
     }
     if (pl.n_serial) {
         Scope sc(profiler, 3, stream);
@@ -389,7 +393,7 @@ int Batch::accel(int mode, bool prepare_only) {
     }
     if (pl.nrows && !fused_np) {
         Scope sc(profiler, 4, stream);
-        launch_np(v, pl.nrows, pl.max_n, pl.maxT, pl.d_blk_recv, pl.n_recv, !field_done, stream);
+        launch_np(v, pl.nrows, pl.max_n, pl.maxT, pl.d_blk_recv, pl.n_recv, !field_done, stream, !items_done);
     }
     return EPH_OK;
 }
