@@ -1653,21 +1653,136 @@ __device__ __noinline__ void stage_rotational_ool(const Acc &A, int g, int k, do
     stage_rotational(A, g, k, dt, dt_k);
 }
 
+// Component c (0, 1, 2 = x, y, z) of the translational stage update: the
+// same operations as stage_translational on that component of every
+// vector (they never mix components), so each thread's results are the
+// bits of the one-thread update.
+__device__ __forceinline__ void stage_translational_c(const View &d, const Acc &A, int g, int c, int k, double dt,
+                                                      double tadd) {
+    const dd v = A.ldd(F_V + 2 * c, g);
+    if (k == 1) {
+        A.std_(F_X0R + 2 * c, g, A.ldd(F_R + 2 * c, g));
+        A.std_(F_X0V + 2 * c, g, v);
+    }
+    const double a_now = (A.ld(F_GACC + c, g) + A.ld(F_DACC + c, g)) + A.ld(F_NACC + c, g);
+    {
+        const int fb = F_STAGE + (k - 1) * kStageFields;
+        A.std_(fb + 2 * c, g, v);
+        A.st(fb + 6 + c, g, a_now);
+    }
+    const dd r0 = A.ldd(F_X0R + 2 * c, g), v0 = A.ldd(F_X0V + 2 * c, g);
+    dd cv = mkdd(0.0);
+    double ca = 0;
+    const int base = k * (k - 1) / 2;
+    const int e1 = c_row0[k + 1];
+#pragma unroll 4
+    for (int e = c_row0[k]; e < e1; ++e) {              // non-zero columns, ascending
+        const int j = c_nzj[e];
+        const double2 ck = c_rk[base + j];
+        const dd ckj{ck.x, ck.y};
+        const double fc = ck.x + ck.y;
+        const int fb = F_STAGE + j * kStageFields;
+        const bool now = j == k - 1;                    // this stage's record, still in registers
+        cv = add(cv, mul(ckj, now ? v : A.ldd(fb + 2 * c, g)));
+        ca += fc * (now ? a_now : A.ld(fb + 6 + c, g));
+    }
+    A.std_(F_R + 2 * c, g, add(r0, mul(mkdd(dt), cv)));
+    A.std_(F_V + 2 * c, g, add(v0, mkdd(dt * ca)));
+    if (c == 0) {
+        const double t = d.t0[d.bsys[g]] + tadd;
+        A.st(F_GM, g, A.ld(F_GM0, g) + A.ld(F_DGM, g) * t);
+        A.st(F_EXJ2, g, A.ld(F_DJ2, g) * t);
+    }
+}
+
 // istep != nullptr: the step index comes from device memory (replayed graphs)
 // and the stage time offset dt * (dt_k + i_step) is formed here, with the
 // same two operations as on the host.
 // Workgroups [0, nblk) take the translational half, [nblk, 2 nblk) the
-// rotational half of the same bodies.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_stage(View d, int k, double dt, double dt_k, double tadd,
-                                              const int64_t *__restrict__ istep) {
+// rotational half of the same 64 bodies; wavefront c of a workgroup takes
+// vector component c, so the long dependent sums over the stage records run
+// on three lanes per body instead of one.  In the rotational half the
+// component sums of the spin meet in LDS and wavefront 0 then forms the
+// rotation of the frame (stage_rotational's operations, in its order).
+__global__ __launch_bounds__(192) void k_stage(View d, int k, double dt, double dt_k, double tadd,
+                                               const int64_t *__restrict__ istep) {
     const Acc A{d.F, d.nbp};
     const int nblk = (d.nb + 63) / 64;
     const bool rot = (int)blockIdx.x >= nblk;
-    const int g = ((int)blockIdx.x - (rot ? nblk : 0)) * 64 + (int)threadIdx.x;
-    if (g >= d.nb) return;
+    const int lane = (int)threadIdx.x & 63, c = (int)threadIdx.x >> 6;
+    const int g = ((int)blockIdx.x - (rot ? nblk : 0)) * 64 + lane;
+    const bool valid = g < d.nb;
     if (istep) tadd = dt * (dt_k + (double)*istep);
-    if (rot) stage_rotational(A, g, k, dt, dt_k);
-    else stage_translational(d, A, g, k, dt, tadd);
+    if (!rot) {
+        if (valid) stage_translational_c(d, A, g, c, k, dt, tadd);
+        return;
+    }
+    __shared__ double cw_s[3][64];
+    const int gg = valid ? g : 0;
+    dd gl_new{0, 0};
+    double cw = 0;
+    // wavefront 0 at the last stage: the inputs of the energy-gradient
+    // correction, read before any wavefront stores this stage's GL
+    m3 GI = mzero();
+    v3 gl_old{0, 0, 0}, w_full{0, 0, 0}, tq_full{0, 0, 0}, egrad{0, 0, 0};
+    if (valid) {
+        if (k == 1) {
+            A.std_(F_X0GL + 2 * c, gg, A.ldd(F_GL + 2 * c, gg));
+            A.stddv(F_X0S + 6 * c, gg, A.lddv(F_S + 6 * c, gg));
+            A.st(F_X0W + c, gg, A.ld(F_W + c, gg));
+            A.st(F_EGRAD + c, gg, A.ld(F_DTQ + c, gg));
+        }
+        const double w_now = A.ld(F_W + c, gg), tq_now = A.ld(F_DTQ + c, gg);
+        {
+            const int fb = F_STAGE + (k - 1) * kStageFields;
+            A.st(fb + 9 + c, gg, w_now);
+            A.st(fb + 12 + c, gg, tq_now);
+        }
+        if (c == 0 && k == 25) {
+            GI = A.ldm(F_GI, gg);
+            gl_old = todbl(A.lddv(F_GL, gg));
+            w_full = A.ldv(F_W, gg);
+            tq_full = A.ldv(F_DTQ, gg);
+            egrad = A.ldv(F_EGRAD, gg);
+        }
+        const dd gl0 = A.ldd(F_X0GL + 2 * c, gg);
+        double ct = 0;
+        const int base = k * (k - 1) / 2;
+        const int e1 = c_row0[k + 1];
+#pragma unroll 4
+        for (int e = c_row0[k]; e < e1; ++e) {          // non-zero columns, ascending
+            const int j = c_nzj[e];
+            const double2 ck = c_rk[base + j];
+            const double fc = ck.x + ck.y;
+            const int fb = F_STAGE + j * kStageFields;
+            const bool now = j == k - 1;
+            cw += fc * (now ? w_now : A.ld(fb + 9 + c, gg));
+            ct += fc * (now ? tq_now : A.ld(fb + 12 + c, gg));
+        }
+        gl_new = add(gl0, mkdd(dt * ct));
+    }
+    cw_s[c][lane] = cw;
+    __syncthreads();
+    if (!valid) return;
+    A.std_(F_GL + 2 * c, gg, gl_new);
+    if (c != 0) return;
+    if (k == 25) {
+        egrad = egrad_k25(GI, gl_old, w_full, tq_full, egrad, dt);
+        A.stv(F_EGRAD, gg, egrad);
+    }
+    // at the first stage the saved start values equal the current ones
+    const v3 w0 = A.ldv(k == 1 ? F_W : F_X0W, gg);
+    ddm3 sm = A.lddm(k == 1 ? F_S : F_X0S, gg);
+    const v3 w = v3{cw_s[0][lane], cw_s[1][lane], cw_s[2][lane]} / dt_k;
+    const v3 dw = (2.0 / 3) * (w - w0);
+    sm = add(sm, rotation_matrix(w - dw, dt * dt_k / 2, DevSin{}) * todbl(sm));
+    sm = add(sm, rotation_matrix(w + dw, dt * dt_k / 2, DevSin{}) * todbl(sm));
+    if (k == 25) {
+        sm = add(sm, rotation_matrix(egrad, 1.0 / 72, DevSin{}) * todbl(sm));
+        sm = orthogonalized(sm);
+    }
+    A.stddm(F_S, gg, sm);
+    A.stv(F_W, gg, w);
 }
 
 __global__ void k_step_count(int64_t *istep) {
@@ -2274,7 +2389,7 @@ void launch_np(const View &d, int nrows, int max_n, int maxT, const int2 *recv_b
     if (nrecv) hipLaunchKernelGGL(k_np_recv, dim3(nrecv), dim3(kBS), 0, st, d, recv_blk, maxT);
 }
 void launch_stage(const View &d, int k, double dt, double dt_k, double tadd, const int64_t *istep, hipStream_t st) {
-    hipLaunchKernelGGL(k_stage, dim3(2 * ((d.nb + 63) / 64)), dim3(64), 0, st, d, k, dt, dt_k, tadd, istep);
+    hipLaunchKernelGGL(k_stage, dim3(2 * ((d.nb + 63) / 64)), dim3(192), 0, st, d, k, dt, dt_k, tadd, istep);
 }
 void launch_step_count(int64_t *istep, hipStream_t st) {
     hipLaunchKernelGGL(k_step_count, dim3(1), dim3(64), 0, st, istep);
