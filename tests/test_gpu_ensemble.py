@@ -91,6 +91,37 @@ def test_xcd_dealt_batch_matches_per_thread_variant(tmp_path_factory):
     assert not bad and bits_equal(ta, tb), bad
 
 
+def test_four_lane_batch_without_time_lag_or_items(tmp_path_factory):
+    """Edge cases of the four-lane launches: a batch with no constant-time-lag
+    body (the deformation launch has only four-lane blocks and item fields)
+    and a batch with no non-point model (no item rows: no item fields in the
+    deformation launch, no item sums in the main-force launch).  Both give
+    the bits of the one-thread-per-body kernels."""
+    specs = gen.make_ensemble(24, seed=95, n_lo=64, n_hi=200)
+    cfgs = ensemble.write_members(specs, tmp_path_factory.mktemp("edge4"))
+    masses, sys_n, gps, rings, t0 = ensemble.load_members(cfgs)
+    assert (masses["tide_delay"] != 0).any() and len(gps) > 0 and len(rings) > 0
+    no_lag = masses.copy()
+    no_lag["tide_delay"] = 0.0
+    no_models = masses.copy()
+    no_models["gp"] = -1
+    no_models["ring"] = -1
+    cases = (("no constant-time-lag body", no_lag, gps, rings),
+             ("no non-point model", no_models, gps[:0], rings[:0]))
+    for label, m, g, r in cases:
+        print(f"\n[case] 24-member batch with {label}: variant 4 vs variant 1, 2 steps", flush=True)
+        k4 = GpuSystems(m, sys_n, g, r, t0)
+        pt = GpuSystems(m, sys_n, g, r, t0)
+        k4.set_variant(4)
+        pt.set_variant(1)
+        k4.integrate(300.0, 2)
+        pt.integrate(300.0, 2)
+        a, ta = k4.download()
+        b, tb = pt.download()
+        bad = [f for f in a.dtype.names if not bits_equal(a[f], b[f])]
+        assert not bad and bits_equal(ta, tb), (label, bad)
+
+
 def test_two_wavefront_fused_batch_matches_per_thread_variant(tmp_path_factory):
     """A batch of six large systems on the one-workgroup-per-body kernels:
     with more than 768 bodies the main force runs two wavefronts per body
